@@ -1,0 +1,139 @@
+/*
+ * wavehip.h -- C ABI of libwavehip.so, the MI355X (gfx950) wavelet engine.
+ *
+ * Two layers:
+ *
+ * 1. The reference package's `.C` entry points, same names, argument
+ *    lists and meaning (reference src/rfunctions.cuh:17-35, bound by
+ *    NAMESPACE:26-30 and called from R/rfunctions.R:394-403, :455-464,
+ *    :596-609).  They take host pointers, copy to the GPU, run the HIP
+ *    kernels and copy back; they never call exit() and report errors on
+ *    stderr, leaving the outputs untouched.
+ *
+ * 2. `wh_*` functions on device pointers with a status return, an
+ *    explicit element type, 64-bit lengths and a caller stream (a
+ *    hipStream_t passed as void*; NULL = the null stream).  Work is
+ *    enqueued asynchronously unless stated otherwise.
+ *
+ * Codes are the reference's: sense FWD=1/BWD=0, ttype DWT=0/MODWT=1
+ * (time ordered), filter Haar=1/D4=2/C6=3/LA8=4, hardness HARD=0/SOFT=1,
+ * threshtype MANUAL=0/UNIV=1/CV=2, levels 0-based for thresholding.
+ *
+ * Layouts (reference src/haar.cpp:9-30, src/daub4.cpp:525-528):
+ *   DWT   in-place interleaved: y[m 2^J] = s_J[m], y[2^{j-1} + m 2^j] = d_j[m]
+ *   MODWT time ordered: xmod[2n(j-1) + i] = S_j[i], xmod[2n(j-1) + n + i] = D_j[i]
+ */
+#ifndef WAVEHIP_H
+#define WAVEHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define WH_OK 0
+#define WH_ERR_ARG 1         /* invalid length / levels / code / pointer */
+#define WH_ERR_HIP 2         /* HIP runtime error (message in wh_last_error) */
+#define WH_ERR_NOMEM 3       /* device allocation failed */
+#define WH_ERR_UNSUPPORTED 4 /* combination not provided */
+#define WH_ERR_NODEVICE 5    /* no usable gfx950 device */
+
+/* element types */
+#define WH_F64 0
+#define WH_F32 1
+
+/* Thresholding specification applied to detail coefficients of the
+   0-based levels minlevel..maxlevel (rule of reference
+   src/thresh.cpp:142-157: |c| < t -> 0, else hard keeps c, soft gives
+   sign(c)(|c|-t)).  `levels`, when not NULL, is a DEVICE array of one
+   threshold per transform level (index = 0-based level) and overrides
+   `value`. */
+typedef struct wh_thresh {
+  int enabled;
+  int hardness;
+  int minlevel;
+  int maxlevel;
+  double value;
+  const double* levels;
+} wh_thresh;
+
+/* ---- library --------------------------------------------------------- */
+
+/* Message of the last error on the calling thread (never NULL). */
+const char* wh_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int wh_version(void);
+/* Optional: select and initialise a device (the current one when < 0). */
+int wh_init(int device);
+/* Resolved level count: reference src/wavutils.cpp:48-65, 0 on error. */
+int wh_resolve_levels(int64_t n, int nlevels, int filter);
+
+/* ---- reference .C entry points (host pointers, synchronous) ----------- */
+
+/* reference src/rfunctions.cuh:22 / src/rfunctions.cu:21-57 */
+void RgpuTransform(double* x, double* xmod, int* len, int* sense, int* nlevels, int* ttype,
+                   int* filter, int* filterlen);
+/* reference src/rfunctions.cuh:34 / src/rfunctions.cu:102-136; all four
+   filters, both transform types and all three threshold types. */
+void RgpuSmooth(double* x, int* len, int* nlevels, int* ttype, int* filter, int* filterlen,
+                int* threshtype, double* thresh, int* hardness, int* minlevel, int* maxlevel,
+                double* tol);
+
+/* ---- device transforms ------------------------------------------------ */
+
+/* MODWT forward: x (n) -> xmod (2 n J).  reference src/daub4.cpp:510-533
+   (time-ordered pattern; C6/LA8 extend it). */
+int wh_modwt_fwd(const void* x, void* xmod, int64_t n, int nlevels, int filter, int dtype,
+                 void* stream);
+/* MODWT inverse: xmod (2 n J) -> x (n), with optional fused thresholding
+   of the detail blocks as they are read.  xmod is not modified.
+   reference src/daub4.cpp:535-589. */
+int wh_modwt_inv(const void* xmod, void* x, int64_t n, int nlevels, int filter, int dtype,
+                 const wh_thresh* thr, void* stream);
+/* DWT forward: x (n) -> y (n), interleaved layout.  y may equal x.
+   reference src/la8.cpp:10-76 (and the Haar/D4/C6 equivalents). */
+int wh_dwt_fwd(const void* x, void* y, int64_t n, int nlevels, int filter, int dtype,
+               void* stream);
+/* DWT inverse: y (n) -> x (n) with optional fused thresholding.  x may
+   equal y.  reference src/la8.cpp:78-115. */
+int wh_dwt_inv(const void* y, void* x, int64_t n, int nlevels, int filter, int dtype,
+               const wh_thresh* thr, void* stream);
+
+/* In-place thresholding of a transformed vector (DWT: n values, MODWT:
+   2 n J values).  reference src/thresh.cpp:15-83. */
+int wh_threshold(void* coeffs, int64_t n, int nlevels, int ttype, int filter, int dtype,
+                 const wh_thresh* thr, void* stream);
+
+/* ---- statistics ------------------------------------------------------- */
+
+/* Exact median / MAD (1.4826 median|x - median x|) of `count` values,
+   result written as one double to the DEVICE pointer `out`.
+   reference src/utils.cpp:166-203. */
+int wh_median(const void* x, int64_t count, int dtype, double* out, void* stream);
+int wh_mad(const void* x, int64_t count, int dtype, double* out, void* stream);
+/* Universal threshold of a transformed vector over 0-based levels
+   [minlevel, maxlevel], one double to DEVICE `out`.
+   reference src/thresh.cpp:170-201. */
+int wh_univ_thresh(const void* coeffs, int64_t n, int nlevels, int ttype, int filter,
+                   int dtype, int minlevel, int maxlevel, double* out, void* stream);
+/* Interpolation error of the CV scheme, one double to DEVICE `out`.
+   reference src/thresh.cpp:344-384. */
+int wh_interp_mse(const void* xn, const void* ye, const void* yo, int64_t n, int dtype,
+                  double* out, void* stream);
+
+/* ---- smoothing -------------------------------------------------------- */
+
+/* Forward transform -> threshold (MANUAL / UNIV / CV) -> inverse, x (device,
+   n values) smoothed in place.  *thresh (host) is the manual threshold on
+   input and receives the chosen one for UNIV / CV.  Synchronous.
+   reference src/rfunctions.cu:71-136, src/thresh.cpp:203-342. */
+int wh_smooth(void* x, int64_t n, int nlevels, int ttype, int filter, int dtype,
+              int threshtype, double* thresh, int hardness, int minlevel, int maxlevel,
+              double tol, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,643 @@
+/*
+ * wvoracle.c -- CPU oracle (TEST INFRASTRUCTURE ONLY; see wvoracle.h).
+ *
+ * A restatement, in plain C, of the reference package's serial CPU
+ * path.  Each function cites the reference lines it follows.  The
+ * floating-point expressions are evaluated in the same order as the
+ * reference's (left-to-right sums of products in the reference's term
+ * order), and the library is built with -ffp-contract=off, so that for
+ * the functions the reference provides the results are bit-identical to
+ * a build of the reference sources with the same flags.
+ *
+ * Parity pins (tests/test_oracle.py): golden vectors written by a build
+ * of the reference's own CPU sources (tests/golden/), impulse known-answer
+ * tests of the convolution alignment, orthogonality of the dense DWT
+ * matrix, Parseval, perfect reconstruction, the MODWT decimation identity
+ * against the DWT, per-level MODWT energy, brute-force medians and the
+ * closed-form threshold rule.
+ */
+#include "wvoracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define O_FWD 1
+#define O_BWD 0
+#define O_DWT 0
+#define O_MODWT 1
+#define O_HARD 0
+#define O_SOFT 1
+#define O_MANUAL 0
+#define O_UNIV 1
+#define O_CV 2
+
+/* Filter constants: the published Haar, Daubechies-4, Coiflet-6 and
+   least-asymmetric-8 scaling filters with the digits the reference uses
+   (src/haarcoeffs.h:4-5, src/daub4coeffs.h:6-9, src/c6coeffs.h:6-11,
+   src/la8coeffs.h:6-13). */
+static const double kInvR2 = 0.70710678118654752440;
+static const double kHInvR2 = 0.35355339059327376220;
+static const double kD4[4] = {0.4829629131445341, 0.8365163037378079, 0.2241438680420134,
+                              -0.1294095225512604};
+static const double kC6[6] = {-7.273261951252645e-2, 3.378976624574818e-1,
+                              8.525720202116004e-1,  3.848648468648577e-1,
+                              -7.273261951252645e-2, -1.565572813579199e-2};
+static const double kLA8[8] = {-7.576571478950221e-2, -2.963552764600249e-2,
+                               4.976186676327750e-1,  8.037387518051321e-1,
+                               2.978577956053061e-1,  -9.921954357663353e-2,
+                               -1.260396726203130e-2, 3.222310060405147e-2};
+
+int wvo_filter_length(int filt) {
+  switch (filt) {
+    case 1: return 2;
+    case 2: return 4;
+    case 3: return 6;
+    case 4: return 8;
+    default: return 0;
+  }
+}
+
+int wvo_filter_coeffs(int filt, double* c) {
+  switch (filt) {
+    case 1: c[0] = kInvR2; c[1] = kInvR2; return 0;
+    case 2: memcpy(c, kD4, sizeof kD4); return 0;
+    case 3: memcpy(c, kC6, sizeof kC6); return 0;
+    case 4: memcpy(c, kLA8, sizeof kLA8); return 0;
+    default: return 1;
+  }
+}
+
+/* g_k = (-1)^k C_{L-1-k}  (src/daub4.cpp:36, src/la8.cpp:43). */
+static void wavelet_filter(const double* c, int L, double* g) {
+  for (int k = 0; k < L; ++k) g[k] = (k & 1) ? -c[L - 1 - k] : c[L - 1 - k];
+}
+
+/* src/wavutils.cpp:48-65 */
+unsigned wvo_check_len_levels(unsigned len, unsigned nlevels, unsigned filtlen) {
+  if (len == 0 || (len & (len - 1)) != 0 || filtlen == 0) return 0;
+  /* the reference casts a negative floor() to unsigned here (undefined);
+     its R layer rejects such lengths (R/rfunctions.R:104-110) */
+  if (len < filtlen) return 0;
+  unsigned maxlevels = (unsigned)floor(log2((double)len / filtlen)) + 1;
+  if (nlevels == 0) nlevels = maxlevels;
+  if (nlevels > maxlevels) return 0;
+  return nlevels;
+}
+
+/* src/wavutils.cpp:67-77 */
+unsigned wvo_check_len_levels_mm(unsigned len, unsigned nlevels, unsigned minlevel,
+                                 unsigned maxlevel, unsigned filtlen) {
+  unsigned r = wvo_check_len_levels(len, nlevels, filtlen);
+  if (r == 0) return 0;
+  if (minlevel > maxlevel || maxlevel >= r) return 0;
+  return r;
+}
+
+/* ------------------------------------------------------------------ DWT */
+
+/* One forward level on the N coefficients a[m] = x[m*skip]
+   (src/la8.cpp:26-76, src/daub4.cpp:20-57, src/c6.cpp:32-78,
+   src/haar.cpp:47-74).  All reads are of pre-level values. */
+static void dwt_fwd_level(double* x, unsigned len, unsigned skip, int filt, const double* c,
+                          const double* g, int L, double* a) {
+  unsigned N = len / skip;
+  for (unsigned m = 0; m < N; ++m) a[m] = x[(size_t)m * skip];
+  for (unsigned m = 0; m < N / 2; ++m) {
+    double s, d;
+    if (filt == 1) {
+      s = (a[2 * m] + a[2 * m + 1]) * kInvR2;
+      d = (a[2 * m] - a[2 * m + 1]) * kInvR2;
+    } else {
+      s = a[2 * m] * c[0];
+      d = a[2 * m] * g[0];
+      for (int k = 1; k < L; ++k) {
+        double v = a[(2 * m + k) & (N - 1)];
+        s = s + v * c[k];
+        d = d + v * g[k];
+      }
+    }
+    x[(size_t)(2 * m) * skip] = s;
+    x[(size_t)(2 * m + 1) * skip] = d;
+  }
+}
+
+/* One inverse level (src/la8.cpp:78-115, src/daub4.cpp:60-88,
+   src/c6.cpp:81-111, src/haar.cpp:76-93).  Even output = sum over
+   q = L/2-1..0 of s[m-q] C_{2q} + d[m-q] g_{2q}; odd output likewise
+   with index 2q+1; terms accumulated in that order. */
+static void dwt_inv_level(double* x, unsigned len, unsigned skip, int filt, const double* c,
+                          const double* g, int L, double* a) {
+  unsigned N = len / skip, P = N / 2;
+  for (unsigned m = 0; m < N; ++m) a[m] = x[(size_t)m * skip];
+  for (unsigned m = 0; m < P; ++m) {
+    double e, o;
+    if (filt == 1) {
+      e = (a[2 * m] + a[2 * m + 1]) * kInvR2;
+      o = (a[2 * m] - a[2 * m + 1]) * kInvR2;
+    } else {
+      int first = 1;
+      e = 0.0;
+      o = 0.0;
+      for (int q = L / 2 - 1; q >= 0; --q) {
+        unsigned mm = (m + P - (unsigned)q) & (P - 1);
+        double sv = a[2 * mm], dv = a[2 * mm + 1];
+        if (first) {
+          e = sv * c[2 * q];
+          o = sv * c[2 * q + 1];
+          first = 0;
+        } else {
+          e = e + sv * c[2 * q];
+          o = o + sv * c[2 * q + 1];
+        }
+        e = e + dv * g[2 * q];
+        o = o + dv * g[2 * q + 1];
+      }
+    }
+    x[(size_t)(2 * m) * skip] = e;
+    x[(size_t)(2 * m + 1) * skip] = o;
+  }
+}
+
+static int dwt_impl(double* x, unsigned len, int sense, unsigned nlevels, int filt, int omp) {
+  (void)omp;
+  int L = wvo_filter_length(filt);
+  if (L == 0 || x == NULL) return 2;
+  unsigned J = wvo_check_len_levels(len, nlevels, (unsigned)L);
+  if (J == 0) return 3;
+  if (sense != O_FWD && sense != O_BWD) return 4;
+  double c[8], g[8];
+  wvo_filter_coeffs(filt, c);
+  wavelet_filter(c, L, g);
+  double* a = (double*)malloc(sizeof(double) * len);
+  if (!a) return 5;
+  if (sense == O_FWD) {
+    for (unsigned j = 0; j < J; ++j) dwt_fwd_level(x, len, 1u << j, filt, c, g, L, a);
+  } else {
+    for (int j = (int)J - 1; j >= 0; --j) dwt_inv_level(x, len, 1u << j, filt, c, g, L, a);
+  }
+  free(a);
+  return 0;
+}
+
+int wvo_dwt(double* x, unsigned len, int sense, unsigned nlevels, int filt) {
+  return dwt_impl(x, len, sense, nlevels, filt, 0);
+}
+
+/* ---------------------------------------------------------------- MODWT */
+
+/* One forward level: S[i] = sum_k C_k X[(i+k s) mod n], D likewise with g
+   (src/daub4.cpp:561-571, src/haar.cpp:590-600). */
+static void modwt_fwd_level(const double* X, double* S, double* D, unsigned len, unsigned s,
+                            int filt, const double* c, const double* g, int L, int omp) {
+  unsigned mask = len - 1;
+  (void)omp;
+#pragma omp parallel for schedule(static) if (omp)
+  for (long long ii = 0; ii < (long long)len; ++ii) {
+    unsigned i = (unsigned)ii;
+    if (filt == 1) {
+      double a = X[i], b = X[(i + s) & mask];
+      D[i] = (a - b) * kInvR2;
+      S[i] = (a + b) * kInvR2;
+    } else {
+      double v = X[i];
+      double sa = v * c[0], da = v * g[0];
+      for (int k = 1; k < L; ++k) {
+        v = X[(i + (unsigned)k * s) & mask];
+        sa = sa + v * c[k];
+        da = da + v * g[k];
+      }
+      S[i] = sa;
+      D[i] = da;
+    }
+  }
+}
+
+/* One inverse level: out[i] = 1/2 sum_k (C_k S[i-k s] + g_k D[i-k s]),
+   terms in the order k = L-2, L-4, ..., 0, L-1, L-3, ..., 1, which is the
+   order of src/daub4.cpp:578-583 (k = 2,0,3,1).  Haar is
+   (S[i]+D[i]+S[i-s]-D[i-s]) / (2 sqrt 2) (src/haar.cpp:602-616). */
+static void modwt_inv_level(double* out, const double* S, const double* D, unsigned len,
+                            unsigned s, int filt, const double* c, const double* g, int L,
+                            int omp) {
+  unsigned mask = len - 1;
+  int order[8], no = 0;
+  for (int k = L - 2; k >= 0; k -= 2) order[no++] = k;
+  for (int k = L - 1; k >= 1; k -= 2) order[no++] = k;
+  (void)omp;
+#pragma omp parallel for schedule(static) if (omp)
+  for (long long ii = 0; ii < (long long)len; ++ii) {
+    unsigned i = (unsigned)ii;
+    if (filt == 1) {
+      unsigned p = (i - s) & mask;
+      out[i] = (S[i] + D[i] + S[p] - D[p]) * kHInvR2;
+    } else {
+      double acc = 0.0;
+      for (int t = 0; t < no; ++t) {
+        int k = order[t];
+        unsigned p = (i - (unsigned)k * s) & mask;
+        if (t == 0)
+          acc = S[p] * c[k];
+        else
+          acc = acc + S[p] * c[k];
+        acc = acc + D[p] * g[k];
+      }
+      out[i] = 0.5 * acc;
+    }
+  }
+}
+
+static int modwt_impl(double* x, double* xmod, unsigned len, int sense, unsigned nlevels,
+                      int filt, int omp) {
+  int L = wvo_filter_length(filt);
+  if (L == 0 || x == NULL || xmod == NULL) return 2;
+  unsigned J = wvo_check_len_levels(len, nlevels, (unsigned)L);
+  if (J == 0) return 3;
+  double c[8], g[8];
+  wvo_filter_coeffs(filt, c);
+  wavelet_filter(c, L, g);
+  size_t n = len;
+  if (sense == O_FWD) {
+    /* src/daub4.cpp:510-533 : level l reads x (l=0) or the S slot of l-1 */
+    for (unsigned l = 0; l < J; ++l) {
+      const double* in = (l == 0) ? x : xmod + 2 * n * (l - 1);
+      modwt_fwd_level(in, xmod + 2 * n * l, xmod + 2 * n * l + n, len, 1u << l, filt, c, g, L,
+                      omp);
+    }
+  } else if (sense == O_BWD) {
+    /* src/daub4.cpp:535-558 : level l writes x (l=0) or the S slot of l-1 */
+    for (int l = (int)J - 1; l >= 0; --l) {
+      double* out = (l == 0) ? x : xmod + 2 * n * (size_t)(l - 1);
+      modwt_inv_level(out, xmod + 2 * n * (size_t)l, xmod + 2 * n * (size_t)l + n, len,
+                      1u << l, filt, c, g, L, omp);
+    }
+  } else {
+    return 4;
+  }
+  return 0;
+}
+
+int wvo_modwt(double* x, double* xmod, unsigned len, int sense, unsigned nlevels, int filt) {
+  return modwt_impl(x, xmod, len, sense, nlevels, filt, 0);
+}
+
+int wvo_modwt_omp(double* x, double* xmod, unsigned len, int sense, unsigned nlevels,
+                  int filt) {
+  return modwt_impl(x, xmod, len, sense, nlevels, filt, 1);
+}
+
+/* Levels of the DWT are short sequences for coarse levels, so the OpenMP
+   variant parallelises the per-level pair loop of the finest levels only. */
+int wvo_dwt_omp(double* x, unsigned len, int sense, unsigned nlevels, int filt) {
+  int L = wvo_filter_length(filt);
+  if (L == 0 || x == NULL) return 2;
+  unsigned J = wvo_check_len_levels(len, nlevels, (unsigned)L);
+  if (J == 0) return 3;
+  double c[8], g[8];
+  wvo_filter_coeffs(filt, c);
+  wavelet_filter(c, L, g);
+  double* a = (double*)malloc(sizeof(double) * len);
+  if (!a) return 5;
+  for (unsigned jj = 0; jj < J; ++jj) {
+    unsigned j = (sense == O_FWD) ? jj : J - 1 - jj;
+    unsigned skip = 1u << j, N = len / skip, P = N / 2;
+#pragma omp parallel for schedule(static)
+    for (long long mm = 0; mm < (long long)N; ++mm) a[mm] = x[(size_t)mm * skip];
+#pragma omp parallel for schedule(static)
+    for (long long mm = 0; mm < (long long)P; ++mm) {
+      unsigned m = (unsigned)mm;
+      double s0, s1;
+      if (filt == 1) {
+        s0 = (a[2 * m] + a[2 * m + 1]) * kInvR2;
+        s1 = (a[2 * m] - a[2 * m + 1]) * kInvR2;
+      } else if (sense == O_FWD) {
+        s0 = a[2 * m] * c[0];
+        s1 = a[2 * m] * g[0];
+        for (int k = 1; k < L; ++k) {
+          double v = a[(2 * m + k) & (N - 1)];
+          s0 = s0 + v * c[k];
+          s1 = s1 + v * g[k];
+        }
+      } else {
+        s0 = 0.0;
+        s1 = 0.0;
+        for (int q = L / 2 - 1; q >= 0; --q) {
+          unsigned r = (m + P - (unsigned)q) & (P - 1);
+          double sv = a[2 * r], dv = a[2 * r + 1];
+          if (q == L / 2 - 1) {
+            s0 = sv * c[2 * q];
+            s1 = sv * c[2 * q + 1];
+          } else {
+            s0 = s0 + sv * c[2 * q];
+            s1 = s1 + sv * c[2 * q + 1];
+          }
+          s0 = s0 + dv * g[2 * q];
+          s1 = s1 + dv * g[2 * q + 1];
+        }
+      }
+      x[(size_t)(2 * m) * skip] = s0;
+      x[(size_t)(2 * m + 1) * skip] = s1;
+    }
+  }
+  free(a);
+  return 0;
+}
+
+/* ------------------------------------------------------------ threshold */
+
+/* src/thresh.cpp:142-157 */
+double wvo_thresh_coef(double coef, double thresh, int hardness) {
+  if (fabs(coef) < thresh) return 0.0;
+  if (hardness == O_SOFT) return ((coef < 0) ? -1 : 1) * (fabs(coef) - thresh);
+  return coef;
+}
+
+/* src/thresh.cpp:15-35 (DWT) and :41-83 (MODWT time ordered). */
+int wvo_threshold(const double* in, double* out, unsigned len, unsigned nlevels, int ttype,
+                  int filt, double thresh, int hardness, unsigned minlevel,
+                  unsigned maxlevel) {
+  int L = wvo_filter_length(filt);
+  if (L == 0) return 2;
+  unsigned J = wvo_check_len_levels_mm(len, nlevels, minlevel, maxlevel, (unsigned)L);
+  if (J == 0) return 3;
+  if (hardness != O_HARD && hardness != O_SOFT) return 4;
+  int cp = (in != out);
+  if (ttype == O_DWT) {
+    if (cp) {
+      /* the scaling coefficients x[m 2^J] are copied as well */
+      for (size_t i = 0; i < len; i += ((size_t)1 << J)) out[i] = in[i];
+    }
+    for (unsigned skip = 1; skip < (1u << J); skip <<= 1) {
+      int active = (skip >= (1u << minlevel)) && (skip <= (1u << maxlevel));
+      for (size_t i = skip; i < len; i += 2 * (size_t)skip) {
+        if (active)
+          out[i] = wvo_thresh_coef(in[i], thresh, hardness);
+        else if (cp)
+          out[i] = in[i];
+      }
+    }
+    return 0;
+  }
+  if (ttype == O_MODWT) {
+    size_t n = len;
+    for (unsigned l = 0; l < J; ++l) {
+      size_t il = 2 * n * l;
+      int active = (l >= minlevel) && (l <= maxlevel);
+      for (size_t i = 0; i < 2 * n; ++i) {
+        if (active && i >= n)
+          out[il + i] = wvo_thresh_coef(in[il + i], thresh, hardness);
+        else if (cp)
+          out[il + i] = in[il + i];
+      }
+    }
+    return 0;
+  }
+  return 5;
+}
+
+/* --------------------------------------------------------- statistics */
+
+static int cmp_double(const void* a, const void* b) {
+  double x = *(const double*)a, y = *(const double*)b;
+  return (x < y) ? -1 : (x > y) ? 1 : 0;
+}
+
+/* src/utils.cpp:166-189 : average of the two middle order statistics. */
+double wvo_median(const double* x, unsigned len) {
+  if (len == 0) return 0.0;
+  double* t = (double*)malloc(sizeof(double) * len);
+  memcpy(t, x, sizeof(double) * len);
+  qsort(t, len, sizeof(double), cmp_double);
+  double med = 0.5 * (t[(len - 1) / 2] + t[len / 2]);
+  free(t);
+  return med;
+}
+
+/* src/utils.cpp:191-203 */
+double wvo_mad(const double* x, unsigned len) {
+  if (len == 0) return 0.0;
+  double med = wvo_median(x, len);
+  double* dev = (double*)malloc(sizeof(double) * len);
+  for (unsigned i = 0; i < len; ++i) dev[i] = fabs(x[i] - med);
+  double m = wvo_median(dev, len);
+  free(dev);
+  return 1.4826 * m;
+}
+
+/* src/thresh.cpp:170-201 with src/wavutils.cpp:683-732 (isolate_dlevels).
+   For the time-ordered MODWT the reference gathers the detail blocks
+   starting at level 0 whatever minlevel is (src/wavutils.cpp:711-717);
+   that is reproduced. */
+double wvo_univ_thresh(const double* x, const double* xmod, unsigned len, unsigned J,
+                       int ttype, unsigned minlevel, unsigned maxlevel) {
+  if (ttype == O_DWT) {
+    if (minlevel == 0 && maxlevel == J - 1)
+      return wvo_mad(x + 1, len - 1) * sqrt(2. * log((double)(len - 1)));
+    unsigned n = 0;
+    for (unsigned l = minlevel; l <= maxlevel; ++l) n += len >> (l + 1);
+    double* t = (double*)malloc(sizeof(double) * (n ? n : 1));
+    unsigned j = 0;
+    for (unsigned skip = 1; skip < (1u << J); skip <<= 1) {
+      if (skip < (1u << minlevel) || skip > (1u << maxlevel)) continue;
+      for (size_t i = skip; i < len; i += 2 * (size_t)skip)
+        if (j < n) t[j++] = x[i];
+    }
+    double ut = wvo_mad(t, n) * sqrt(2. * log((double)len));
+    free(t);
+    return ut;
+  }
+  size_t n = (size_t)(maxlevel - minlevel + 1) * len;
+  double* t = (double*)malloc(sizeof(double) * n);
+  for (unsigned l = 0; l <= maxlevel - minlevel; ++l)
+    memcpy(t + (size_t)l * len, xmod + (size_t)len * (2 * l + 1), sizeof(double) * len);
+  double ut = wvo_mad(t, (unsigned)n) * sqrt(2. * log((double)len));
+  free(t);
+  return ut;
+}
+
+/* src/thresh.cpp:344-384 */
+double wvo_interp_mse(const double* xn, const double* ye, const double* yo, unsigned len) {
+  unsigned lenh = len / 2, j = 0;
+  double m = 0.0, yie, yio;
+  for (unsigned i = 0; i < len; i += 2) {
+    if (i == 0) {
+      yie = yo[0];
+      yio = 0.5 * (ye[j] + ye[j + 1]);
+    } else if (i == len - 2) {
+      yio = ye[lenh - 1];
+      yie = 0.5 * (yo[j] + yo[j - 1]);
+    } else {
+      yio = 0.5 * (ye[j] + ye[j + 1]);
+      yie = 0.5 * (yo[j] + yo[j - 1]);
+    }
+    j++;
+    m += (yie - xn[i]) * (yie - xn[i]) + (yio - xn[i + 1]) * (yio - xn[i + 1]);
+  }
+  return m * 0.5;
+}
+
+/* A transform "object" for the CV bookkeeping (src/wvtheads.h:40-50). */
+typedef struct {
+  double* x;
+  double* xmod;
+  unsigned len, levels;
+  int ttype, filt;
+} owst;
+
+static int ow_init(owst* w, unsigned len, unsigned levels, int ttype, int filt) {
+  w->len = len;
+  w->levels = levels;
+  w->ttype = ttype;
+  w->filt = filt;
+  w->x = (double*)calloc(len, sizeof(double));
+  w->xmod = NULL;
+  if (ttype == O_MODWT) w->xmod = (double*)calloc((size_t)2 * len * levels, sizeof(double));
+  return (w->x == NULL || (ttype == O_MODWT && w->xmod == NULL)) ? 1 : 0;
+}
+static void ow_free(owst* w) {
+  free(w->x);
+  free(w->xmod);
+}
+static int ow_transform(owst* w, int sense) {
+  if (w->ttype == O_DWT) return wvo_dwt(w->x, w->len, sense, w->levels, w->filt);
+  return wvo_modwt(w->x, w->xmod, w->len, sense, w->levels, w->filt);
+}
+static int ow_threshold(const owst* in, owst* out, double t, int hardness, unsigned mn,
+                        unsigned mx) {
+  const double* src = (in->ttype == O_DWT) ? in->x : in->xmod;
+  double* dst = (out->ttype == O_DWT) ? out->x : out->xmod;
+  return wvo_threshold(src, dst, in->len, in->levels, in->ttype, in->filt, t, hardness, mn, mx);
+}
+
+static void put_trace(double* trace, int cap, int* nt, double t0, double t1, double t2,
+                      double t3, double m1, double m2) {
+  if (trace == NULL || *nt >= cap) return;
+  double* r = trace + 6 * (*nt);
+  r[0] = t0; r[1] = t1; r[2] = t2; r[3] = t3; r[4] = m1; r[5] = m2;
+  (*nt)++;
+}
+
+/* src/thresh.cpp:203-342 : Nason's two-fold cross validation with a
+   golden-section search on [0, universal threshold].
+   Deviation: when minlevel == maxlevel the reference thresholds the
+   half-length transforms over an empty level range and then reads
+   buffers it never wrote (src/thresh.cpp:262 with :113-140); that case
+   is rejected here (status 6).  For the DWT the out-of-place threshold
+   copies the coarse scaling coefficients x[m 2^J], m >= 1, which the
+   reference leaves unwritten (src/thresh.cpp:15-35); with the Haar filter
+   at full depth no such coefficient exists and the two agree. */
+double wvo_cvt(double* x, unsigned len, unsigned nlevels, int ttype, int filt, int hardness,
+               double tol, unsigned minlevel, unsigned maxlevel, double* trace,
+               int trace_cap, int* ntrace, int* status) {
+  const double R = 0.61803399, C = 1. - R;
+  int nt = 0, st = 0;
+  double result = 0.0;
+  owst w, we, wo, y1e, y1o, y2e, y2o;
+  int L = wvo_filter_length(filt);
+  unsigned J = (L > 0) ? wvo_check_len_levels_mm(len, nlevels, minlevel, maxlevel, (unsigned)L)
+                       : 0;
+  if (ntrace) *ntrace = 0;
+  if (J == 0) { st = 3; goto done_nofree; }
+  if (J <= 1) { st = 7; goto done_nofree; }
+  if (minlevel == maxlevel) { st = 6; goto done_nofree; }
+  unsigned lenh = len >> 1, maxlevel1 = maxlevel - 1;
+  ow_init(&w, len, J, ttype, filt);
+  ow_init(&we, lenh, J - 1, ttype, filt);
+  ow_init(&wo, lenh, J - 1, ttype, filt);
+  ow_init(&y1e, lenh, J - 1, ttype, filt);
+  ow_init(&y1o, lenh, J - 1, ttype, filt);
+  ow_init(&y2e, lenh, J - 1, ttype, filt);
+  ow_init(&y2o, lenh, J - 1, ttype, filt);
+  for (unsigned i = 0; i < lenh; ++i) {
+    we.x[i] = x[2 * i];
+    wo.x[i] = x[2 * i + 1];
+  }
+  memcpy(w.x, x, sizeof(double) * len);
+  ow_transform(&w, O_FWD);
+  double tc = wvo_univ_thresh(w.x, w.xmod, len, J, ttype, minlevel, maxlevel);
+  double tb = tc / 2.;
+  double t0 = 0., t2 = tb, t1 = tb - C * (tb - 0.), t3 = tc;
+  ow_transform(&we, O_FWD);
+  ow_transform(&wo, O_FWD);
+  ow_threshold(&we, &y1e, t1, hardness, minlevel, maxlevel1);
+  ow_threshold(&wo, &y1o, t1, hardness, minlevel, maxlevel1);
+  ow_threshold(&we, &y2e, t2, hardness, minlevel, maxlevel1);
+  ow_threshold(&wo, &y2o, t2, hardness, minlevel, maxlevel1);
+  ow_transform(&y1e, O_BWD);
+  ow_transform(&y1o, O_BWD);
+  ow_transform(&y2e, O_BWD);
+  ow_transform(&y2o, O_BWD);
+  double m1 = wvo_interp_mse(x, y1e.x, y1o.x, len);
+  double m2 = wvo_interp_mse(x, y2e.x, y2o.x, len);
+  put_trace(trace, trace_cap, &nt, t0, t1, t2, t3, m1, m2);
+  unsigned iter = 0;
+  while (fabs(t3 - t0) > tol * (t1 + t2)) {
+    if (iter > 50) break;
+    if (m2 < m1) {
+      t0 = t1; t1 = t2; t2 = R * t2 + C * t3;
+      m1 = m2;
+      ow_threshold(&we, &y2e, t2, hardness, minlevel, maxlevel1);
+      ow_threshold(&wo, &y2o, t2, hardness, minlevel, maxlevel1);
+      ow_transform(&y2e, O_BWD);
+      ow_transform(&y2o, O_BWD);
+      m2 = wvo_interp_mse(x, y2e.x, y2o.x, len);
+    } else {
+      t3 = t2; t2 = t1; t1 = R * t1 + C * t0;
+      m2 = m1;
+      ow_threshold(&we, &y1e, t1, hardness, minlevel, maxlevel1);
+      ow_threshold(&wo, &y1o, t1, hardness, minlevel, maxlevel1);
+      ow_transform(&y1e, O_BWD);
+      ow_transform(&y1o, O_BWD);
+      m1 = wvo_interp_mse(x, y1e.x, y1o.x, len);
+    }
+    iter++;
+    put_trace(trace, trace_cap, &nt, t0, t1, t2, t3, m1, m2);
+  }
+  tc = m1 < m2 ? t1 : t2;
+  tc = tc / (sqrt(1. - log(2.) / log((double)len)));
+  ow_threshold(&w, &w, tc, hardness, minlevel, maxlevel);
+  ow_transform(&w, O_BWD);
+  memcpy(x, w.x, sizeof(double) * len);
+  result = tc;
+  ow_free(&w); ow_free(&we); ow_free(&wo);
+  ow_free(&y1e); ow_free(&y1o); ow_free(&y2e); ow_free(&y2o);
+done_nofree:
+  if (ntrace) *ntrace = nt;
+  if (status) *status = st;
+  return result;
+}
+
+/* src/rfunctions.cu:71-100 (RcpuSmooth). */
+int wvo_smooth(double* x, unsigned len, unsigned nlevels, int ttype, int filt, int threshtype,
+               double* thresh, int hardness, unsigned minlevel, unsigned maxlevel,
+               double tol) {
+  int L = wvo_filter_length(filt);
+  if (L == 0 || x == NULL) return 2;
+  unsigned J = wvo_check_len_levels(len, nlevels, (unsigned)L);
+  if (J == 0) return 3;
+  if (threshtype == O_CV) {
+    int st = 0;
+    double t = wvo_cvt(x, len, J, ttype, filt, hardness, tol, minlevel, maxlevel, NULL, 0, NULL,
+                       &st);
+    if (st == 0 && thresh) *thresh = t;
+    return st;
+  }
+  if (threshtype != O_UNIV && threshtype != O_MANUAL) return 8;
+  if (wvo_check_len_levels_mm(len, J, minlevel, maxlevel, (unsigned)L) == 0) return 3;
+  owst w;
+  if (ow_init(&w, len, J, ttype, filt)) return 5;
+  memcpy(w.x, x, sizeof(double) * len);
+  ow_transform(&w, O_FWD);
+  double t;
+  if (threshtype == O_UNIV) {
+    t = wvo_univ_thresh(w.x, w.xmod, len, J, ttype, 0, 0);
+    if (thresh) *thresh = t;
+  } else {
+    t = thresh ? *thresh : 0.0;
+  }
+  ow_threshold(&w, &w, t, hardness, minlevel, maxlevel);
+  ow_transform(&w, O_BWD);
+  memcpy(x, w.x, sizeof(double) * len);
+  ow_free(&w);
+  return 0;
+}

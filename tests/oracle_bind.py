@@ -1,0 +1,186 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) and, where it was
+built, of the reference's own CPU sources (oracle/_ref/libwvref.so).
+
+Test infrastructure: used by tests/, the smoke check and bench.py's CPU
+baseline only, as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_SO = ROOT / "oracle" / "liboracle.so"
+REF_SO = ROOT / "oracle" / "_ref" / "libwvref.so"
+
+_DP = C.POINTER(C.c_double)
+_U = C.c_uint
+_I = C.c_int
+
+_o = None
+_r = None
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_DP)
+
+
+def oracle():
+    global _o
+    if _o is None:
+        if not ORACLE_SO.exists():
+            subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)
+        L = C.CDLL(str(ORACLE_SO))
+        L.wvo_check_len_levels.restype = _U
+        L.wvo_check_len_levels.argtypes = [_U, _U, _U]
+        L.wvo_check_len_levels_mm.restype = _U
+        L.wvo_check_len_levels_mm.argtypes = [_U, _U, _U, _U, _U]
+        L.wvo_filter_coeffs.argtypes = [_I, _DP]
+        L.wvo_dwt.argtypes = [_DP, _U, _I, _U, _I]
+        L.wvo_dwt_omp.argtypes = [_DP, _U, _I, _U, _I]
+        L.wvo_modwt.argtypes = [_DP, _DP, _U, _I, _U, _I]
+        L.wvo_modwt_omp.argtypes = [_DP, _DP, _U, _I, _U, _I]
+        L.wvo_thresh_coef.restype = C.c_double
+        L.wvo_thresh_coef.argtypes = [C.c_double, C.c_double, _I]
+        L.wvo_threshold.argtypes = [_DP, _DP, _U, _U, _I, _I, C.c_double, _I, _U, _U]
+        L.wvo_median.restype = C.c_double
+        L.wvo_median.argtypes = [_DP, _U]
+        L.wvo_mad.restype = C.c_double
+        L.wvo_mad.argtypes = [_DP, _U]
+        L.wvo_univ_thresh.restype = C.c_double
+        L.wvo_univ_thresh.argtypes = [_DP, _DP, _U, _U, _I, _U, _U]
+        L.wvo_interp_mse.restype = C.c_double
+        L.wvo_interp_mse.argtypes = [_DP, _DP, _DP, _U]
+        L.wvo_cvt.restype = C.c_double
+        L.wvo_cvt.argtypes = [_DP, _U, _U, _I, _I, _I, C.c_double, _U, _U, _DP, _I,
+                              C.POINTER(_I), C.POINTER(_I)]
+        L.wvo_smooth.argtypes = [_DP, _U, _U, _I, _I, _I, _DP, _I, _U, _U, C.c_double]
+        _o = L
+    return _o
+
+
+def ref_available() -> bool:
+    return REF_SO.exists()
+
+
+def ref():
+    global _r
+    if _r is None:
+        L = C.CDLL(str(REF_SO))
+        L.ref_transform.argtypes = [_DP, _DP, _I, _I, _I, _I, _I, _I]
+        L.ref_threshold.argtypes = [_DP, _DP, _I, _I, _I, _I, _I, C.c_double, _I, _I, _I]
+        L.ref_smooth.argtypes = [_DP, _I, _I, _I, _I, _I, _I, _DP, _I, _I, _I, C.c_double]
+        L.ref_median.restype = C.c_double
+        L.ref_median.argtypes = [_DP, _I]
+        L.ref_mad.restype = C.c_double
+        L.ref_mad.argtypes = [_DP, _I]
+        L.ref_univ_thresh.restype = C.c_double
+        L.ref_univ_thresh.argtypes = [_DP, _DP, _I, _I, _I, _I, _I, _I, _I]
+        L.ref_interp_mse.restype = C.c_double
+        L.ref_interp_mse.argtypes = [_DP, _DP, _DP, _I]
+        L.ref_check_len_levels.restype = _U
+        L.ref_check_len_levels.argtypes = [_U, _U, _U]
+        _r = L
+    return _r
+
+
+FLEN = {1: 2, 2: 4, 3: 6, 4: 8}
+
+
+def levels(n, nlevels, filt):
+    return oracle().wvo_check_len_levels(n, nlevels, FLEN[filt])
+
+
+# ---- numpy conveniences over the oracle ---------------------------------
+
+def dwt(x, nlevels, filt, sense=1, omp=False):
+    y = np.array(x, dtype=np.float64, copy=True)
+    fn = oracle().wvo_dwt_omp if omp else oracle().wvo_dwt
+    rc = fn(_p(y), len(y), sense, nlevels, filt)
+    assert rc == 0, rc
+    return y
+
+
+def modwt_fwd(x, nlevels, filt, omp=False):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = len(x)
+    J = levels(n, nlevels, filt)
+    xm = np.zeros(2 * n * J)
+    fn = oracle().wvo_modwt_omp if omp else oracle().wvo_modwt
+    rc = fn(_p(x), _p(xm), n, 1, J, filt)
+    assert rc == 0, rc
+    return xm
+
+
+def modwt_inv(xmod, n, nlevels, filt, omp=False):
+    J = levels(n, nlevels, filt)
+    xm = np.array(xmod[: 2 * n * J], dtype=np.float64, copy=True)
+    x = np.zeros(n)
+    fn = oracle().wvo_modwt_omp if omp else oracle().wvo_modwt
+    rc = fn(_p(x), _p(xm), n, 0, J, filt)
+    assert rc == 0, rc
+    return x
+
+
+def threshold(coeffs, n, nlevels, ttype, filt, t, hardness, minlevel, maxlevel):
+    c = np.array(coeffs, dtype=np.float64, copy=True)
+    rc = oracle().wvo_threshold(_p(c), _p(c), n, nlevels, ttype, filt, t, hardness, minlevel,
+                                maxlevel)
+    assert rc == 0, rc
+    return c
+
+
+def median(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return oracle().wvo_median(_p(x), len(x))
+
+
+def mad(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return oracle().wvo_mad(_p(x), len(x))
+
+
+def univ_thresh(coeffs, n, nlevels, ttype, minlevel, maxlevel):
+    c = np.ascontiguousarray(coeffs, dtype=np.float64)
+    if ttype == 0:
+        return oracle().wvo_univ_thresh(_p(c), None, n, nlevels, 0, minlevel, maxlevel)
+    return oracle().wvo_univ_thresh(None, _p(c), n, nlevels, 1, minlevel, maxlevel)
+
+
+def interp_mse(xn, ye, yo):
+    xn, ye, yo = (np.ascontiguousarray(a, dtype=np.float64) for a in (xn, ye, yo))
+    return oracle().wvo_interp_mse(_p(xn), _p(ye), _p(yo), len(xn))
+
+
+def cvt(x, nlevels, ttype, filt, hardness, tol, minlevel, maxlevel, trace_cap=64):
+    y = np.array(x, dtype=np.float64, copy=True)
+    tr = np.zeros(6 * trace_cap)
+    nt = C.c_int(0)
+    st = C.c_int(0)
+    t = oracle().wvo_cvt(_p(y), len(y), nlevels, ttype, filt, hardness, tol, minlevel, maxlevel,
+                         _p(tr), trace_cap, C.byref(nt), C.byref(st))
+    return t, y, tr[: 6 * nt.value].reshape(-1, 6), st.value
+
+
+def smooth(x, nlevels, ttype, filt, threshtype, thresh, hardness, minlevel, maxlevel, tol=0.01):
+    y = np.array(x, dtype=np.float64, copy=True)
+    t = np.array([thresh if thresh is not None else 0.0])
+    rc = oracle().wvo_smooth(_p(y), len(y), nlevels, ttype, filt, threshtype, _p(t), hardness,
+                             minlevel, maxlevel, tol)
+    return y, float(t[0]), rc
+
+
+def relerr(a, b):
+    """Norm-wise relative error max|a-b| / max|b|."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.max(np.abs(b))
+    if den == 0:
+        return float(np.max(np.abs(a - b)))
+    return float(np.max(np.abs(a - b)) / den)

@@ -1,0 +1,138 @@
+"""GPU parity of the transforms against the CPU oracle (through the C ABI).
+
+Bar (north star): FP64 results within 1e-12 norm-wise relative
+(max|a-b| / max|b|, per level block) of the oracle; FP32 within 1e-5 of
+the FP64 oracle; perfect reconstruction to the same bounds.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle_bind as O
+from wavecuda_amd import device as D
+from wavecuda_amd.signals import noisy_doppler, white_noise
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 1e-5
+FILTERS = [1, 2, 3, 4]
+
+
+def _dev(a, dtype=torch.float64):
+    return torch.tensor(a, dtype=dtype, device="cuda")
+
+
+def _blocks_relerr(got, ref, n):
+    """Max over the n-long blocks of the norm-wise relative error."""
+    got = np.asarray(got)
+    ref = np.asarray(ref)
+    return max(O.relerr(got[i:i + n], ref[i:i + n]) for i in range(0, len(ref), n))
+
+
+def _dwt_level_relerr(got, ref, J):
+    """Max over DWT levels (and the coarse scaling block) of the error."""
+    n = len(ref)
+    errs = []
+    for j in range(1, J + 1):
+        idx = np.arange(1 << (j - 1), n, 1 << j)
+        errs.append(O.relerr(got[idx], ref[idx]))
+    idx = np.arange(0, n, 1 << J)
+    errs.append(O.relerr(got[idx], ref[idx]))
+    return max(errs)
+
+
+MODWT_CASES = [(8, 0), (16, 0), (64, 2), (256, 0), (1024, 5), (4096, 0), (1 << 14, 10),
+               (1 << 16, 0), (1 << 18, 10)]
+
+
+@pytest.mark.parametrize("filt", FILTERS)
+@pytest.mark.parametrize("n,nlev", MODWT_CASES)
+def test_modwt_parity_f64(filt, n, nlev):
+    if O.levels(n, nlev, filt) == 0:
+        pytest.skip("length/levels invalid for this filter")
+    J = O.levels(n, nlev, filt)
+    x = white_noise(n, seed=n + filt)
+    ref = O.modwt_fwd(x, J, filt)
+    got = D.modwt_fwd(_dev(x), J, filt)
+    torch.cuda.synchronize()
+    assert _blocks_relerr(got.cpu().numpy(), ref, n) <= TOL64
+    # inverse of the oracle's coefficients
+    xr_ref = O.modwt_inv(ref, n, J, filt)
+    xr = D.modwt_inv(_dev(ref), n, J, filt).cpu().numpy()
+    assert O.relerr(xr, xr_ref) <= TOL64
+    assert O.relerr(xr, x) <= TOL64
+
+
+@pytest.mark.parametrize("filt", FILTERS)
+@pytest.mark.parametrize("n,nlev", MODWT_CASES[2:])
+def test_modwt_parity_f32(filt, n, nlev):
+    J = O.levels(n, nlev, filt)
+    x, _ = noisy_doppler(n, 5.0, seed=3)
+    ref = O.modwt_fwd(x, J, filt)
+    got = D.modwt_fwd(_dev(x, torch.float32), J, filt).cpu().numpy()
+    assert _blocks_relerr(got, ref, n) <= TOL32
+    xr = D.modwt_inv(_dev(ref, torch.float32), n, J, filt).cpu().numpy()
+    assert O.relerr(xr, x) <= TOL32
+
+
+DWT_CASES = [(2, 0), (4, 0), (8, 0), (16, 0), (64, 0), (1024, 0), (1024, 3), (8192, 0),
+             (16384, 0), (16384, 7), (1 << 16, 0), (1 << 16, 4), (1 << 18, 0), (1 << 20, 12)]
+
+
+@pytest.mark.parametrize("filt", FILTERS)
+@pytest.mark.parametrize("n,nlev", DWT_CASES)
+def test_dwt_parity_f64(filt, n, nlev):
+    if O.levels(n, nlev, filt) == 0:
+        pytest.skip("length/levels invalid for this filter")
+    J = O.levels(n, nlev, filt)
+    x, _ = noisy_doppler(n, 5.0, seed=n)
+    ref = O.dwt(x, J, filt, sense=1)
+    got = D.dwt_fwd(_dev(x), J, filt).cpu().numpy()
+    assert _dwt_level_relerr(got, ref, J) <= TOL64
+    xr_ref = O.dwt(ref, J, filt, sense=0)
+    xr = D.dwt_inv(_dev(ref), J, filt).cpu().numpy()
+    assert O.relerr(xr, xr_ref) <= TOL64
+    assert O.relerr(xr, x) <= TOL64
+
+
+@pytest.mark.parametrize("filt", FILTERS)
+@pytest.mark.parametrize("n,nlev", [(1024, 0), (16384, 0), (1 << 18, 0)])
+def test_dwt_parity_f32(filt, n, nlev):
+    J = O.levels(n, nlev, filt)
+    x = white_noise(n, seed=7)
+    ref = O.dwt(x, J, filt, sense=1)
+    got = D.dwt_fwd(_dev(x, torch.float32), J, filt).cpu().numpy()
+    assert _dwt_level_relerr(got, ref, J) <= TOL32
+    xr = D.dwt_inv(_dev(ref, torch.float32), J, filt).cpu().numpy()
+    assert O.relerr(xr, x) <= TOL32
+
+
+@pytest.mark.parametrize("filt", FILTERS)
+def test_dwt_in_place(filt):
+    n = 1 << 16
+    x = white_noise(n, seed=11)
+    J = O.levels(n, 0, filt)
+    t = _dev(x)
+    D.dwt_fwd(t, J, filt, out=t)
+    ref = O.dwt(x, J, filt)
+    assert O.relerr(t.cpu().numpy(), ref) <= TOL64
+    D.dwt_inv(t, J, filt, out=t)
+    assert O.relerr(t.cpu().numpy(), x) <= TOL64
+
+
+@pytest.mark.parametrize("filt", FILTERS)
+@pytest.mark.parametrize("kind", ["dwt", "modwt"])
+def test_large_perfect_reconstruction(filt, kind):
+    """n = 2^24 (the benchmark size): size-independent PR property."""
+    n = 1 << 24
+    x = torch.randn(n, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(5))
+    if kind == "dwt":
+        y = D.dwt_fwd(x, 0, filt)
+        xr = D.dwt_inv(y, 0, filt)
+    else:
+        y = D.modwt_fwd(x, 10, filt)
+        xr = D.modwt_inv(y, n, 10, filt)
+        del y
+    err = (xr - x).abs().max().item() / x.abs().max().item()
+    assert err <= TOL64

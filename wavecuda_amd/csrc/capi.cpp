@@ -1,0 +1,261 @@
+// capi.cpp -- the C ABI of libwavehip.so: errors, device state, scratch,
+// the wh_* device entry points and the reference's .C entry points.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "wh_common.h"
+#include "wh_internal.h"
+
+namespace wh {
+
+namespace {
+thread_local std::string g_err = "no error";
+std::mutex g_mu;
+
+constexpr int kMaxDev = 64;
+DeviceInfo g_info[kMaxDev];
+struct Scratch {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+Scratch g_scratch[kMaxDev][2];
+
+int current_device() {
+  int d = -1;
+  if (hipGetDevice(&d) != hipSuccess) return -1;
+  return d;
+}
+
+int scratch_impl(int which, size_t bytes, void** out) {
+  int d = current_device();
+  if (d < 0 || d >= kMaxDev) return set_error(WH_ERR_NODEVICE, "no current HIP device");
+  std::lock_guard<std::mutex> lk(g_mu);
+  Scratch& s = g_scratch[d][which];
+  if (s.bytes < bytes) {
+    if (s.p) {
+      // work already queued may still read the old buffer
+      if (hipDeviceSynchronize() != hipSuccess) return set_error(WH_ERR_HIP, "sync before scratch grow");
+      (void)hipFree(s.p);
+      s.p = nullptr;
+      s.bytes = 0;
+    }
+    size_t want = bytes + (bytes >> 3) + 4096;
+    if (hipMalloc(&s.p, want) != hipSuccess) {
+      s.p = nullptr;
+      return set_error(WH_ERR_NOMEM, "scratch allocation failed");
+    }
+    s.bytes = want;
+  }
+  *out = s.p;
+  return WH_OK;
+}
+}  // namespace
+
+int set_error(int code, const char* msg) {
+  g_err = msg ? msg : "error";
+  return code;
+}
+
+int set_hip_error(hipError_t e, const char* what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return WH_ERR_HIP;
+}
+
+const DeviceInfo& device_info() {
+  static DeviceInfo none;
+  int d = current_device();
+  if (d < 0 || d >= kMaxDev) return none;
+  std::lock_guard<std::mutex> lk(g_mu);
+  DeviceInfo& di = g_info[d];
+  if (di.device != d) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, d) == hipSuccess) {
+      di.num_cus = prop.multiProcessorCount;
+      di.max_lds_per_block = prop.sharedMemPerBlock;
+      di.warp_size = prop.warpSize;
+      di.arch = prop.gcnArchName;
+    }
+    if (di.num_cus <= 0) di.num_cus = 256;
+    if (di.max_lds_per_block == 0) di.max_lds_per_block = 65536;
+    di.device = d;
+  }
+  return di;
+}
+
+int scratch_get(size_t bytes, void** out) { return scratch_impl(0, bytes, out); }
+int scratch2_get(size_t bytes, void** out) { return scratch_impl(1, bytes, out); }
+
+}  // namespace wh
+
+using namespace wh;
+
+namespace {
+
+bool valid_filter(int f) { return f >= kHAAR && f <= kLA8; }
+bool valid_dtype(int t) { return t == WH_F64 || t == WH_F32; }
+
+int check_common(const void* a, const void* b, int64_t n, int nlevels, int filter, int dtype,
+                 int* J) {
+  if (!a || !b) return set_error(WH_ERR_ARG, "null pointer");
+  if (!valid_filter(filter)) return set_error(WH_ERR_ARG, "filter must be 1 (Haar), 2 (D4), 3 (C6) or 4 (LA8)");
+  if (!valid_dtype(dtype)) return set_error(WH_ERR_ARG, "dtype must be WH_F64 or WH_F32");
+  *J = resolve_levels(n, nlevels, filter_length(filter));
+  if (*J <= 0) return set_error(WH_ERR_ARG, "length must be a power of two and nlevels <= floor(log2(n/L))+1");
+  return WH_OK;
+}
+
+int check_thresh(const wh_thresh* thr, int J) {
+  if (!thr || !thr->enabled) return WH_OK;
+  if (thr->hardness != kHARD && thr->hardness != kSOFT) return set_error(WH_ERR_ARG, "hardness must be 0 (hard) or 1 (soft)");
+  if (thr->minlevel < 0 || thr->minlevel > thr->maxlevel || thr->maxlevel >= J)
+    return set_error(WH_ERR_ARG, "threshold levels must satisfy 0 <= minlevel <= maxlevel < nlevels");
+  return WH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* wh_last_error(void) { return g_err.c_str(); }
+
+int wh_version(void) { return 100; }
+
+int wh_init(int device) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    return set_error(WH_ERR_NODEVICE, "no HIP device");
+  if (device >= 0) {
+    if (device >= count) return set_error(WH_ERR_ARG, "device index out of range");
+    WH_HIP_CHECK(hipSetDevice(device));
+  }
+  const DeviceInfo& di = device_info();
+  if (di.arch.rfind("gfx950", 0) != 0) {
+    std::string m = "device arch " + di.arch + " is not gfx950";
+    return set_error(WH_ERR_NODEVICE, m.c_str());
+  }
+  return WH_OK;
+}
+
+int wh_resolve_levels(int64_t n, int nlevels, int filter) {
+  if (!valid_filter(filter)) return 0;
+  return resolve_levels(n, nlevels, filter_length(filter));
+}
+
+int wh_modwt_fwd(const void* x, void* xmod, int64_t n, int nlevels, int filter, int dtype,
+                 void* stream) {
+  int J, rc = check_common(x, xmod, n, nlevels, filter, dtype, &J);
+  if (rc) return rc;
+  return modwt_forward(x, xmod, n, J, filter, dtype, static_cast<hipStream_t>(stream));
+}
+
+int wh_modwt_inv(const void* xmod, void* x, int64_t n, int nlevels, int filter, int dtype,
+                 const wh_thresh* thr, void* stream) {
+  int J, rc = check_common(xmod, x, n, nlevels, filter, dtype, &J);
+  if (rc) return rc;
+  if ((rc = check_thresh(thr, J))) return rc;
+  return modwt_inverse(xmod, x, n, J, filter, dtype, thr, static_cast<hipStream_t>(stream));
+}
+
+int wh_dwt_fwd(const void* x, void* y, int64_t n, int nlevels, int filter, int dtype,
+               void* stream) {
+  int J, rc = check_common(x, y, n, nlevels, filter, dtype, &J);
+  if (rc) return rc;
+  return dwt_forward(x, y, n, J, filter, dtype, static_cast<hipStream_t>(stream));
+}
+
+int wh_dwt_inv(const void* y, void* x, int64_t n, int nlevels, int filter, int dtype,
+               const wh_thresh* thr, void* stream) {
+  int J, rc = check_common(y, x, n, nlevels, filter, dtype, &J);
+  if (rc) return rc;
+  if ((rc = check_thresh(thr, J))) return rc;
+  return dwt_inverse(y, x, n, J, filter, dtype, thr, static_cast<hipStream_t>(stream));
+}
+
+// Diagnostic: the level-group plan chosen for a MODWT (written to buf).
+int wh_debug_modwt_plan(int inverse, int64_t n, int nlevels, int filter, int dtype, char* buf,
+                        int cap) {
+  int J = resolve_levels(n, nlevels, filter_length(filter));
+  if (J <= 0 || !buf || cap <= 0) return WH_ERR_ARG;
+  std::string s = modwt_plan_string(inverse != 0, n, J, filter, dtype);
+  snprintf(buf, size_t(cap), "%s", s.c_str());
+  return WH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// reference .C entry points (src/rfunctions.cuh:17-35)
+
+static void report(const char* where, int rc) {
+  fprintf(stderr, "\n Error in GPU %s: %s (code %d)\n", where, wh_last_error(), rc);
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t bytes) {
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      p = nullptr;
+      return set_error(WH_ERR_NOMEM, "device allocation failed");
+    }
+    return WH_OK;
+  }
+};
+
+void RgpuTransform(double* x, double* xmod, int* len, int* sense, int* nlevels, int* ttype,
+                   int* filter, int* filterlen) {
+  (void)filterlen;
+  if (!x || !len || !sense || !nlevels || !ttype || !filter) {
+    report("transform", set_error(WH_ERR_ARG, "null argument"));
+    return;
+  }
+  const int64_t n = *len;
+  const int J = valid_filter(*filter) ? resolve_levels(n, *nlevels, filter_length(*filter)) : 0;
+  if (J <= 0) {
+    report("transform", set_error(WH_ERR_ARG, "invalid length, levels or filter"));
+    return;
+  }
+  int rc = WH_OK;
+  if (*ttype == kDWT) {
+    DevBuf a, b;
+    if ((rc = a.alloc(n * 8)) || (rc = b.alloc(n * 8))) return report("transform", rc);
+    if (hipMemcpy(a.p, x, n * 8, hipMemcpyHostToDevice) != hipSuccess)
+      return report("transform", set_error(WH_ERR_HIP, "copy in"));
+    rc = (*sense == kFWD) ? wh_dwt_fwd(a.p, b.p, n, J, *filter, WH_F64, nullptr)
+                          : wh_dwt_inv(a.p, b.p, n, J, *filter, WH_F64, nullptr, nullptr);
+    if (rc) return report("transform", rc);
+    if (hipMemcpy(x, b.p, n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+      return report("transform", set_error(WH_ERR_HIP, "copy out"));
+    return;
+  }
+  if (*ttype == kMODWT) {
+    if (!xmod) return report("transform", set_error(WH_ERR_ARG, "null xmod"));
+    const size_t nm = size_t(2) * size_t(n) * size_t(J);
+    DevBuf a, b;
+    if ((rc = a.alloc(n * 8)) || (rc = b.alloc(nm * 8))) return report("transform", rc);
+    if (*sense == kFWD) {
+      if (hipMemcpy(a.p, x, n * 8, hipMemcpyHostToDevice) != hipSuccess)
+        return report("transform", set_error(WH_ERR_HIP, "copy in"));
+      if ((rc = wh_modwt_fwd(a.p, b.p, n, J, *filter, WH_F64, nullptr))) return report("transform", rc);
+      if (hipMemcpy(xmod, b.p, nm * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return report("transform", set_error(WH_ERR_HIP, "copy out"));
+    } else {
+      if (hipMemcpy(b.p, xmod, nm * 8, hipMemcpyHostToDevice) != hipSuccess)
+        return report("transform", set_error(WH_ERR_HIP, "copy in"));
+      if ((rc = wh_modwt_inv(b.p, a.p, n, J, *filter, WH_F64, nullptr, nullptr)))
+        return report("transform", rc);
+      if (hipMemcpy(x, a.p, n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return report("transform", set_error(WH_ERR_HIP, "copy out"));
+    }
+    return;
+  }
+  report("transform", set_error(WH_ERR_ARG, "ttype must be 0 (DWT) or 1 (MODWT)"));
+}
+
+}  // extern "C"

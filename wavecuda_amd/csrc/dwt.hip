@@ -1,0 +1,550 @@
+// dwt.hip -- level-fused periodic DWT, forward and inverse (gfx950).
+//
+// Math and layout follow the reference's CPU convolution transforms
+// (src/haar.cpp:32-93, src/daub4.cpp:5-88, src/c6.cpp:15-111,
+// src/la8.cpp:10-115): level j (s = 2^{j-1}) maps the coefficients
+// a[m] = y[m s] of the previous level to
+//   s_j[m] = sum_k C_k a[(2m+k) mod N],  d_j[m] = sum_k g_k a[(2m+k) mod N]
+// stored in place at y[2m s] / y[(2m+1) s] (convolution alignment, not the
+// lifting alignment of the reference's GPU code), and the inverse is the
+// transpose: a[2m] = sum_q s[m-q] C_{2q} + d[m-q] g_{2q}, a[2m+1] likewise.
+//
+// Kernel design.  The transform runs as a few passes over compact
+// sequences.  A tile pass fuses g <= 5 levels: each workgroup loads a tile
+// of TILE coefficients plus the (2^g - 1)(L - 2) halo its cascade needs
+// (forward; the inverse needs at most L - 2 per level below the tile),
+// computes all g levels in LDS, writes the details to their interleaved
+// positions of y (fully coalesced for the first pass) and hands the coarse
+// s on as a compact array, so the next pass reads contiguous memory.  Once
+// the compact sequence fits one workgroup's LDS a single "whole" kernel
+// finishes every remaining level.  Thresholding is fused into the inverse
+// as the details are read.
+#include <algorithm>
+#include <vector>
+
+#include "wh_common.h"
+#include "wh_internal.h"
+
+namespace wh {
+
+namespace {
+
+constexpr int kTile = 2048;   // coefficients per tile (level j0 units)
+constexpr int kNT = 256;      // threads per tile workgroup
+constexpr int kNTW = 1024;    // threads of the whole-sequence workgroup
+constexpr int kGmax = 5;      // levels per tile pass
+
+__host__ __device__ constexpr int halo_fwd(int levels, int L) { return ((1 << levels) - 1) * (L - 2); }
+
+struct Thr {
+  int on, hardness, minlevel, maxlevel;
+  double value;
+  const double* levels;
+};
+
+template <typename T>
+__device__ __forceinline__ T thr_level(const Thr& t, int lev0) {
+  if (!t.on || lev0 < t.minlevel || lev0 > t.maxlevel) return T(-1);
+  return t.levels ? T(t.levels[lev0]) : T(t.value);
+}
+
+// One forward pair (s, d) from src[i0 .. i0+L-1] (indices already reduced).
+template <typename T, int FILT>
+__device__ __forceinline__ void fwd_pair(const T* v, T& s, T& d) {
+  constexpr int L = Filter<FILT>::L;
+  if constexpr (FILT == kHAAR) {
+    s = (v[0] + v[1]) * T(kInvR2);
+    d = (v[0] - v[1]) * T(kInvR2);
+  } else {
+    s = v[0] * T(Filter<FILT>::c(0));
+    d = v[0] * T(gcoef<FILT>(0));
+#pragma unroll
+    for (int k = 1; k < L; ++k) {
+      s = fma(v[k], T(Filter<FILT>::c(k)), s);
+      d = fma(v[k], T(gcoef<FILT>(k)), d);
+    }
+  }
+}
+
+// One inverse pair from sv[q], dv[q] = s[m-q], d[m-q], q = 0..L/2-1.
+template <typename T, int FILT>
+__device__ __forceinline__ void inv_pair(const T* sv, const T* dv, T& e, T& o) {
+  constexpr int H = Filter<FILT>::L / 2;
+  if constexpr (FILT == kHAAR) {
+    e = (sv[0] + dv[0]) * T(kInvR2);
+    o = (sv[0] - dv[0]) * T(kInvR2);
+  } else {
+    e = sv[H - 1] * T(Filter<FILT>::c(2 * (H - 1)));
+    o = sv[H - 1] * T(Filter<FILT>::c(2 * (H - 1) + 1));
+    e = fma(dv[H - 1], T(gcoef<FILT>(2 * (H - 1))), e);
+    o = fma(dv[H - 1], T(gcoef<FILT>(2 * (H - 1) + 1)), o);
+#pragma unroll
+    for (int q = H - 2; q >= 0; --q) {
+      e = fma(sv[q], T(Filter<FILT>::c(2 * q)), e);
+      o = fma(sv[q], T(Filter<FILT>::c(2 * q + 1)), o);
+      e = fma(dv[q], T(gcoef<FILT>(2 * q)), e);
+      o = fma(dv[q], T(gcoef<FILT>(2 * q + 1)), o);
+    }
+  }
+}
+
+struct TileArgs {
+  const void* src;   // compact input (fwd: s_{j0}; inv: s_{j0+g} at src[i << src_shift])
+  void* y;           // interleaved coefficient array
+  void* dst;         // fwd: compact s_{j0+g}; inv: compact s_{j0}
+  int64_t n;         // full length
+  int64_t N;         // compact length at level j0
+  int j0, g;
+  int final_pass;    // fwd: s_{j0+g} = s_J goes to y
+  int write_coarse;  // fwd: write y[q] with ctz >= g (first pass only)
+  int src_shift;
+  Thr thr;
+};
+
+// ------------------------------------------------------------ forward tile
+template <typename T, int FILT>
+__global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
+  constexpr int L = Filter<FILT>::L;
+  constexpr int HMAX = halo_fwd(kGmax, L);
+  constexpr int SZA = kTile + HMAX;
+  constexpr int SZB = kTile / 2 + halo_fwd(kGmax - 1, L);
+  __shared__ T bufA[SZA];
+  __shared__ T bufB[SZB];
+  __shared__ T dbuf[kTile];
+
+  const int tid = threadIdx.x;
+  const int g = a.g;
+  const int64_t N = a.N;
+  const int64_t base = int64_t(blockIdx.x) * kTile;
+  const T* __restrict__ src = static_cast<const T*>(a.src);
+  T* __restrict__ y = static_cast<T*>(a.y);
+
+  const int K0 = kTile + halo_fwd(g, L);
+  for (int q = tid; q < K0; q += kNT) bufA[q] = src[(base + q) & (N - 1)];
+  __syncthreads();
+
+  T* cur = bufA;
+  T* nxt = bufB;
+  for (int l = 1; l <= g; ++l) {
+    const int own = kTile >> l;
+    const int Kl = own + halo_fwd(g - l, L);
+    const int doff = kTile - (kTile >> (l - 1));
+    for (int i = tid; i < Kl; i += kNT) {
+      T v[L];
+#pragma unroll
+      for (int k = 0; k < L; ++k) v[k] = cur[2 * i + k];
+      T s, d;
+      fwd_pair<T, FILT>(v, s, d);
+      nxt[i] = s;
+      if (i < own) dbuf[doff + i] = d;
+    }
+    __syncthreads();
+    T* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  // cur holds s_{j0+g}; own part is cur[0 .. kTile >> g)
+  const int j0 = a.j0;
+  for (int q = tid; q < kTile; q += kNT) {
+    int z = (q == 0) ? 31 : __builtin_ctz(unsigned(q));
+    T v;
+    if (z >= g) {
+      if (!(a.final_pass || a.write_coarse)) continue;
+      v = cur[q >> g];
+    } else {
+      v = dbuf[(kTile - (kTile >> z)) + (q >> (z + 1))];
+    }
+    y[(base + q) << j0] = v;
+  }
+  if (!a.final_pass) {
+    T* __restrict__ dst = static_cast<T*>(a.dst);
+    const int own = kTile >> g;
+    for (int i = tid; i < own; i += kNT) dst[(base >> g) + i] = cur[i];
+  }
+}
+
+// ------------------------------------------------------------ inverse tile
+template <typename T, int FILT>
+__global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
+  constexpr int L = Filter<FILT>::L;
+  constexpr int HP = L / 2 - 1;
+  constexpr int PADS = 8;
+  __shared__ T bufX[kTile + PADS];
+  __shared__ T bufY[kTile / 2 + PADS];
+  __shared__ T dbuf[kTile + PADS * kGmax];
+
+  const int tid = threadIdx.x;
+  const int g = a.g, j0 = a.j0;
+  const int64_t N = a.N;
+  const int64_t base = int64_t(blockIdx.x) * kTile;
+  const T* __restrict__ src = static_cast<const T*>(a.src);
+  const T* __restrict__ y = static_cast<const T*>(a.y);
+
+  // Halo below the tile at level l: the inverse of level l+1 needs s_l and
+  // d_l from h_l = e_{l-1}/2 + L/2 - 1 positions below the tile; every
+  // level-l buffer starts e_l = h_l rounded up to even positions below it,
+  // so a pair's two outputs never straddle a buffer start.  e_l <= L - 2.
+  auto evn = [](int l) {
+    int ee = 0;
+    for (int i = 1; i <= l; ++i) ee = ((ee / 2 + HP) + 1) & ~1;
+    return ee;
+  };
+  // details d_l over local [0, own_l + e_l), global m = B_l - e_l + i
+  int doff = 0;
+  for (int l = 1; l <= g; ++l) {
+    const int own = kTile >> l;
+    const int el = evn(l);
+    const int cnt = own + el;
+    const int64_t Nl = N >> l;
+    const int64_t Bl = base >> l;
+    const int jg = j0 + l;  // global 1-based level
+    const T t = thr_level<T>(a.thr, jg - 1);
+    for (int i = tid; i < cnt; i += kNT) {
+      int64_t m = (Bl - el + i) & (Nl - 1);
+      T d = y[(2 * m + 1) << (jg - 1)];
+      if (t >= T(0)) d = thresh_coef(d, t, a.thr.hardness);
+      dbuf[doff + i] = d;
+    }
+    doff += own + PADS;
+  }
+  // top s_{j0+g}, same range as d_g
+  {
+    const int own = kTile >> g;
+    const int eg = evn(g);
+    const int cnt = own + eg;
+    const int64_t Ng = N >> g;
+    const int64_t Bg = base >> g;
+    T* top = (g & 1) ? bufY : bufX;  // level g reads the buffer level g+1 would write
+    for (int i = tid; i < cnt; i += kNT) {
+      int64_t m = (Bg - eg + i) & (Ng - 1);
+      top[i] = src[m << a.src_shift];
+    }
+  }
+  __syncthreads();
+  // levels g..1: level l reads s_l, d_l (start B_l - e_l) and writes
+  // s_{l-1} over local [0, own_{l-1} + e_{l-1}) (start B_{l-1} - e_{l-1})
+  for (int l = g; l >= 1; --l) {
+    const T* cur = (l & 1) ? bufY : bufX;
+    T* out = (l & 1) ? bufX : bufY;
+    int dof = 0;
+    for (int ll = 1; ll < l; ++ll) dof += (kTile >> ll) + PADS;
+    const T* dl = dbuf + dof;
+    const int elm = evn(l - 1);
+    const int npairs = ((kTile >> (l - 1)) + elm) / 2;
+    // pair p has global index m = B_l - e_{l-1}/2 + p, local e_l - e_{l-1}/2 + p
+    const int off = evn(l) - elm / 2;
+    for (int p = tid; p < npairs; p += kNT) {
+      T sv[L / 2], dv[L / 2];
+#pragma unroll
+      for (int q = 0; q < L / 2; ++q) {
+        sv[q] = cur[off + p - q];
+        dv[q] = dl[off + p - q];
+      }
+      T e0, o0;
+      inv_pair<T, FILT>(sv, dv, e0, o0);
+      out[2 * p] = e0;
+      out[2 * p + 1] = o0;
+    }
+    __syncthreads();
+  }
+  // bufX holds s_{j0} for local [0, kTile)
+  T* __restrict__ dst = static_cast<T*>(a.dst);
+  for (int q = tid; q < kTile; q += kNT) dst[base + q] = bufX[q];
+}
+
+// ----------------------------------------------------------- whole kernels
+struct WholeArgs {
+  const void* src;  // fwd: compact s_{j0} (N values); inv: y
+  void* y;
+  void* dst;        // inv: compact s_{j0}
+  int64_t N;        // compact length at level j0
+  int j0, J;
+  Thr thr;
+};
+
+template <typename T, int FILT>
+__global__ __launch_bounds__(kNTW) void dwt_fwd_whole(WholeArgs a) {
+  constexpr int L = Filter<FILT>::L;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* A = reinterpret_cast<T*>(smem_raw);
+  T* B = A + a.N;
+  const int tid = threadIdx.x;
+  const T* __restrict__ src = static_cast<const T*>(a.src);
+  T* __restrict__ y = static_cast<T*>(a.y);
+  const int N = int(a.N);
+  for (int i = tid; i < N; i += kNTW) A[i] = src[i];
+  __syncthreads();
+  T* cur = A;
+  T* nxt = B;
+  int Nl = N;
+  for (int j = a.j0 + 1; j <= a.J; ++j) {
+    const int P = Nl / 2;
+    for (int m = tid; m < P; m += kNTW) {
+      T v[L];
+#pragma unroll
+      for (int k = 0; k < L; ++k) v[k] = cur[(2 * m + k) & (Nl - 1)];
+      T s, d;
+      fwd_pair<T, FILT>(v, s, d);
+      nxt[m] = s;
+      y[int64_t(2 * m + 1) << (j - 1)] = d;
+    }
+    __syncthreads();
+    T* t = cur;
+    cur = nxt;
+    nxt = t;
+    Nl = P;
+  }
+  for (int m = tid; m < Nl; m += kNTW) y[int64_t(m) << a.J] = cur[m];
+}
+
+template <typename T, int FILT>
+__global__ __launch_bounds__(kNTW) void dwt_inv_whole(WholeArgs a) {
+  constexpr int L = Filter<FILT>::L;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* A = reinterpret_cast<T*>(smem_raw);
+  T* B = A + a.N;
+  const int tid = threadIdx.x;
+  const T* __restrict__ y = static_cast<const T*>(a.src);
+  T* __restrict__ dst = static_cast<T*>(a.dst);
+  const int N = int(a.N);
+  const int j0 = a.j0, J = a.J;
+  // s_J into the buffer that level J reads: output of level j goes to A
+  // when (j - j0) is odd, B otherwise; level J reads the output slot of J+1
+  int NJ = N >> (J - j0);
+  T* top = ((J + 1 - j0) & 1) ? A : B;
+  for (int m = tid; m < NJ; m += kNTW) top[m] = y[int64_t(m) << J];
+  __syncthreads();
+  int Nl = NJ;  // length of s_j
+  for (int j = J; j > j0; --j) {
+    const T* cur = ((j + 1 - j0) & 1) ? A : B;
+    T* out = ((j - j0) & 1) ? A : B;
+    const T t = thr_level<T>(a.thr, j - 1);
+    for (int m = tid; m < Nl; m += kNTW) {
+      T sv[L / 2], dv[L / 2];
+#pragma unroll
+      for (int q = 0; q < L / 2; ++q) {
+        int mm = (m - q) & (Nl - 1);
+        sv[q] = cur[mm];
+        T d = y[int64_t(2 * mm + 1) << (j - 1)];
+        if (t >= T(0)) d = thresh_coef(d, t, a.thr.hardness);
+        dv[q] = d;
+      }
+      T ev, ov;
+      inv_pair<T, FILT>(sv, dv, ev, ov);
+      out[2 * m] = ev;
+      out[2 * m + 1] = ov;
+    }
+    __syncthreads();
+    Nl *= 2;
+  }
+  // the output of level j0+1 is in A
+  for (int i = tid; i < N; i += kNTW) dst[i] = A[i];
+}
+
+// ------------------------------------------------------------------- host
+struct Pass {
+  bool whole;
+  int j0, g;
+  int64_t N;
+};
+
+// Largest compact length finished by one workgroup: (N + N/2) values
+// must fit the LDS budget.
+template <typename T>
+int64_t whole_max() {
+  const size_t budget = std::min<size_t>(device_info().max_lds_per_block, 160 * 1024);
+  int64_t m = 1;
+  while (m * 2 <= 16384 && size_t(m * 2 + m) * sizeof(T) <= budget) m *= 2;
+  return m;
+}
+
+template <typename KERN>
+void allow_big_lds(KERN kern) {
+  static bool done = false;
+  if (!done) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    done = true;
+  }
+}
+
+template <typename T>
+std::vector<Pass> plan_passes(int64_t n, int J) {
+  std::vector<Pass> ps;
+  int j0 = 0;
+  int64_t N = n;
+  while (j0 < J) {
+    if (N <= whole_max<T>() || N < int64_t(kTile)) {
+      ps.push_back({true, j0, J - j0, N});
+      break;
+    }
+    int g = std::min(kGmax, J - j0);
+    ps.push_back({false, j0, g, N});
+    j0 += g;
+    N >>= g;
+  }
+  return ps;
+}
+
+Thr thr_of(const wh_thresh* t) {
+  Thr r{};
+  if (t && t->enabled) {
+    r.on = 1;
+    r.hardness = t->hardness;
+    r.minlevel = t->minlevel;
+    r.maxlevel = t->maxlevel;
+    r.value = t->value;
+    r.levels = t->levels;
+  }
+  return r;
+}
+
+template <typename T, int FILT>
+int dwt_fwd_f(const void* x, void* y, int64_t n, int J, hipStream_t st) {
+  auto ps = plan_passes<T>(n, J);
+  const T* src = static_cast<const T*>(x);
+  // in place: the first pass reads halos of neighbouring tiles, so stage x
+  if (x == y && !ps.front().whole) {
+    void* p = nullptr;
+    int rc = scratch2_get(size_t(n) * sizeof(T), &p);
+    if (rc) return rc;
+    WH_HIP_CHECK(hipMemcpyAsync(p, x, size_t(n) * sizeof(T), hipMemcpyDeviceToDevice, st));
+    src = static_cast<const T*>(p);
+  }
+  T* comp = nullptr;
+  if (ps.size() > 1) {
+    void* p = nullptr;
+    int rc = scratch_get(size_t(n >> kGmax) * 2 * sizeof(T) + 64, &p);
+    if (rc) return rc;
+    comp = static_cast<T*>(p);
+  }
+  int slot = 0;
+  for (size_t pi = 0; pi < ps.size(); ++pi) {
+    const Pass& P = ps[pi];
+    if (P.whole) {
+      WholeArgs a{};
+      a.src = src;
+      a.y = y;
+      a.N = P.N;
+      a.j0 = P.j0;
+      a.J = J;
+      size_t lds = size_t(P.N + P.N / 2) * sizeof(T);
+      allow_big_lds(dwt_fwd_whole<T, FILT>);
+      hipLaunchKernelGGL((dwt_fwd_whole<T, FILT>), dim3(1), dim3(kNTW), lds, st, a);
+      WH_HIP_CHECK(hipGetLastError());
+    } else {
+      TileArgs a{};
+      a.src = src;
+      a.y = y;
+      T* out = comp + slot * (n >> kGmax);
+      a.dst = out;
+      a.n = n;
+      a.N = P.N;
+      a.j0 = P.j0;
+      a.g = P.g;
+      a.final_pass = (P.j0 + P.g == J);
+      a.write_coarse = (P.j0 == 0);
+      int64_t blocks = P.N / kTile;
+      hipLaunchKernelGGL((dwt_fwd_tile<T, FILT>), dim3(unsigned(blocks)), dim3(kNT), 0, st, a);
+      WH_HIP_CHECK(hipGetLastError());
+      src = out;
+      slot ^= 1;
+    }
+  }
+  return WH_OK;
+}
+
+template <typename T, int FILT>
+int dwt_inv_f(const void* yin, void* x, int64_t n, int J, const wh_thresh* thr,
+              hipStream_t st) {
+  auto ps = plan_passes<T>(n, J);
+  const T* y = static_cast<const T*>(yin);
+  if (yin == x && !ps.front().whole) {
+    void* p = nullptr;
+    int rc = scratch2_get(size_t(n) * sizeof(T), &p);
+    if (rc) return rc;
+    WH_HIP_CHECK(hipMemcpyAsync(p, yin, size_t(n) * sizeof(T), hipMemcpyDeviceToDevice, st));
+    y = static_cast<const T*>(p);
+  }
+  T* comp = nullptr;
+  if (ps.size() > 1) {
+    void* p = nullptr;
+    int rc = scratch_get(size_t(n >> kGmax) * 2 * sizeof(T) + 64, &p);
+    if (rc) return rc;
+    comp = static_cast<T*>(p);
+  }
+  const Thr th = thr_of(thr);
+  const T* src = nullptr;  // compact s handed down between passes
+  int slot = 0;
+  for (int pi = int(ps.size()) - 1; pi >= 0; --pi) {
+    const Pass& P = ps[pi];
+    T* out = (P.j0 == 0) ? static_cast<T*>(x) : comp + slot * (n >> kGmax);
+    if (P.whole) {
+      WholeArgs a{};
+      a.src = y;
+      a.dst = out;
+      a.N = P.N;
+      a.j0 = P.j0;
+      a.J = J;
+      a.thr = th;
+      size_t lds = size_t(P.N + P.N / 2) * sizeof(T);
+      allow_big_lds(dwt_inv_whole<T, FILT>);
+      hipLaunchKernelGGL((dwt_inv_whole<T, FILT>), dim3(1), dim3(kNTW), lds, st, a);
+      WH_HIP_CHECK(hipGetLastError());
+    } else {
+      TileArgs a{};
+      const bool top = (P.j0 + P.g == J);
+      a.src = top ? static_cast<const void*>(y) : static_cast<const void*>(src);
+      a.src_shift = top ? J : 0;
+      a.y = const_cast<T*>(y);
+      a.dst = out;
+      a.n = n;
+      a.N = P.N;
+      a.j0 = P.j0;
+      a.g = P.g;
+      a.thr = th;
+      int64_t blocks = P.N / kTile;
+      hipLaunchKernelGGL((dwt_inv_tile<T, FILT>), dim3(unsigned(blocks)), dim3(kNT), 0, st, a);
+      WH_HIP_CHECK(hipGetLastError());
+    }
+    src = out;
+    slot ^= 1;
+  }
+  return WH_OK;
+}
+
+template <typename T>
+int dwt_fwd_t(const void* x, void* y, int64_t n, int J, int filt, hipStream_t st) {
+  switch (filt) {
+    case kHAAR: return dwt_fwd_f<T, kHAAR>(x, y, n, J, st);
+    case kD4: return dwt_fwd_f<T, kD4>(x, y, n, J, st);
+    case kC6: return dwt_fwd_f<T, kC6>(x, y, n, J, st);
+    default: return dwt_fwd_f<T, kLA8>(x, y, n, J, st);
+  }
+}
+
+template <typename T>
+int dwt_inv_t(const void* y, void* x, int64_t n, int J, int filt, const wh_thresh* thr,
+              hipStream_t st) {
+  switch (filt) {
+    case kHAAR: return dwt_inv_f<T, kHAAR>(y, x, n, J, thr, st);
+    case kD4: return dwt_inv_f<T, kD4>(y, x, n, J, thr, st);
+    case kC6: return dwt_inv_f<T, kC6>(y, x, n, J, thr, st);
+    default: return dwt_inv_f<T, kLA8>(y, x, n, J, thr, st);
+  }
+}
+
+}  // namespace
+
+int dwt_forward(const void* x, void* y, int64_t n, int J, int filt, int dtype, hipStream_t st) {
+  return dtype == WH_F32 ? dwt_fwd_t<float>(x, y, n, J, filt, st)
+                         : dwt_fwd_t<double>(x, y, n, J, filt, st);
+}
+
+int dwt_inverse(const void* y, void* x, int64_t n, int J, int filt, int dtype,
+                const wh_thresh* thr, hipStream_t st) {
+  return dtype == WH_F32 ? dwt_inv_t<float>(y, x, n, J, filt, thr, st)
+                         : dwt_inv_t<double>(y, x, n, J, filt, thr, st);
+}
+
+}  // namespace wh

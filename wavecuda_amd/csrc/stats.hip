@@ -1,0 +1,374 @@
+// stats.hip -- thresholding and the smoothing statistics (gfx950).
+//
+//  * in-place hard/soft thresholding of DWT / MODWT details
+//    (reference src/thresh.cpp:15-83, :142-157);
+//  * exact median and MAD (reference src/utils.cpp:166-203) by MSB-first
+//    radix selection over order-preserving integer keys of the values:
+//    each pass builds an LDS-privatised histogram of one 11-bit digit of
+//    the keys that still match the prefix found so far, and a one-block
+//    scan picks the bucket holding the wanted rank.  Both middle order
+//    statistics are tracked at once; results are bit-exact (a median is an
+//    order statistic; the final 0.5(a+b) and 1.4826 m are the reference's
+//    operations);
+//  * the universal threshold (reference src/thresh.cpp:170-201), whose
+//    sqrt(2 ln n) factor is computed on the host by the C library so it
+//    rounds like the reference;
+//  * the CV interpolation error (reference src/thresh.cpp:344-384) as a
+//    fixed-order two-stage reduction, so repeated runs are bit-identical.
+#include <cmath>
+
+#include "wh_common.h"
+#include "wh_internal.h"
+#include "wh_stats.h"
+
+namespace wh {
+
+// ---------------------------------------------------------------- threshold
+template <typename T>
+__global__ void thresh_dwt_kernel(T* __restrict__ y, int64_t n, int J, Thr1 th) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (i == 0) continue;
+    const int l = __builtin_ctzll(uint64_t(i));  // detail of 0-based level l
+    if (l >= J || l < th.minlevel || l > th.maxlevel) continue;
+    const T t = th.levels ? T(th.levels[l]) : T(th.value);
+    y[i] = thresh_coef(y[i], t, th.hardness);
+  }
+}
+
+template <typename T>
+__global__ void thresh_modwt_kernel(T* __restrict__ xmod, int64_t n, Thr1 th) {
+  const int64_t cnt = int64_t(th.maxlevel - th.minlevel + 1) * n;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < cnt; e += stride) {
+    const int l = th.minlevel + int(e / n);
+    const int64_t i = 2 * n * l + n + (e - int64_t(l - th.minlevel) * n);
+    const T t = th.levels ? T(th.levels[l]) : T(th.value);
+    xmod[i] = thresh_coef(xmod[i], t, th.hardness);
+  }
+}
+
+int threshold_inplace(void* c, int64_t n, int J, int ttype, int dtype, const wh_thresh* thr,
+                      hipStream_t st) {
+  Thr1 th{thr->hardness, thr->minlevel, thr->maxlevel, thr->value, thr->levels};
+  const int nt = 256;
+  const int64_t work = (ttype == kDWT) ? n : int64_t(thr->maxlevel - thr->minlevel + 1) * n;
+  const unsigned blocks = unsigned(std::min<int64_t>((work + nt - 1) / nt, 4096));
+  if (ttype == kDWT) {
+    if (dtype == WH_F32)
+      hipLaunchKernelGGL(thresh_dwt_kernel<float>, dim3(blocks), dim3(nt), 0, st,
+                         static_cast<float*>(c), n, J, th);
+    else
+      hipLaunchKernelGGL(thresh_dwt_kernel<double>, dim3(blocks), dim3(nt), 0, st,
+                         static_cast<double*>(c), n, J, th);
+  } else {
+    if (dtype == WH_F32)
+      hipLaunchKernelGGL(thresh_modwt_kernel<float>, dim3(blocks), dim3(nt), 0, st,
+                         static_cast<float*>(c), n, th);
+    else
+      hipLaunchKernelGGL(thresh_modwt_kernel<double>, dim3(blocks), dim3(nt), 0, st,
+                         static_cast<double*>(c), n, th);
+  }
+  WH_HIP_CHECK(hipGetLastError());
+  return WH_OK;
+}
+
+// ------------------------------------------------------------ radix select
+
+// order-preserving keys (larger value -> larger unsigned key)
+__device__ __forceinline__ uint64_t okey(double v) {
+  uint64_t u = __double_as_longlong(v);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double from_okey(uint64_t k) {
+  uint64_t u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __longlong_as_double(int64_t(u));
+}
+__device__ __forceinline__ uint64_t okey(float v) {
+  uint32_t u = __float_as_uint(v);
+  return (u >> 31) ? uint64_t(~u) : uint64_t(u | 0x80000000u);
+}
+
+template <typename T>
+__device__ __forceinline__ T value_from_key(uint64_t k);
+template <>
+__device__ __forceinline__ double value_from_key<double>(uint64_t k) {
+  return from_okey(k);
+}
+template <>
+__device__ __forceinline__ float value_from_key<float>(uint64_t k) {
+  uint32_t kk = uint32_t(k);
+  uint32_t u = (kk >> 31) ? (kk & 0x7fffffffu) : ~kk;
+  return __uint_as_float(u);
+}
+
+// Element e of the selection source (see SelSrc).  Returns false when the
+// position is not part of the selection (mode 2 filters by level).
+template <typename T>
+__device__ __forceinline__ bool sel_value(const SelSrc& s, int64_t e, T& v) {
+  const T* __restrict__ b = static_cast<const T*>(s.base);
+  if (s.mode == 0) {
+    v = b[s.off + e];
+    return true;
+  }
+  if (s.mode == 1) {  // MODWT detail blocks 0..nblocks-1
+    const int64_t blk = e / s.n;
+    v = b[2 * s.n * blk + s.n + (e - blk * s.n)];
+    return true;
+  }
+  // mode 2: DWT details of 0-based levels [lmin, lmax] (position e, e >= 1)
+  if (e == 0) return false;
+  const int l = __builtin_ctzll(uint64_t(e));
+  if (l >= s.J || l < s.lmin || l > s.lmax) return false;
+  v = b[e];
+  return true;
+}
+
+constexpr int kBits = 11;
+constexpr int kBins = 1 << kBits;
+
+__global__ void sel_init_kernel(SelState* st, int64_t count) {
+  const int t = threadIdx.x;
+  for (int i = t; i < 2 * kBins; i += blockDim.x) st->hist[i] = 0;
+  if (t == 0) {
+    st->prefix[0] = st->prefix[1] = 0;
+    st->k[0] = (count - 1) / 2;
+    st->k[1] = count / 2;
+  }
+}
+
+// pass over all elements: histogram of digit [lo, hi) of the keys whose
+// bits above hi equal the current prefix of rank r (r = 0, 1)
+template <typename T>
+__global__ __launch_bounds__(256) void sel_hist_kernel(SelSrc s, SelState* st, int hi, int lo,
+                                                       int deviation) {
+  __shared__ unsigned int h[2][kBins];
+  for (int i = threadIdx.x; i < 2 * kBins; i += blockDim.x) (&h[0][0])[i] = 0;
+  __syncthreads();
+  const uint64_t p0 = st->prefix[0], p1 = st->prefix[1];
+  const bool same = (p0 == p1);
+  const T med = deviation ? T(st->med) : T(0);
+  const int64_t E = (s.mode == 2) ? s.n : s.count;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  const uint64_t dmask = (uint64_t(1) << (hi - lo)) - 1;
+  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < E; e += stride) {
+    T v;
+    if (!sel_value<T>(s, e, v)) continue;
+    if (deviation) v = fabs(v - med);
+    const uint64_t k = okey(v);
+    const uint64_t top = (hi >= 64) ? 0 : (k >> hi);
+    const unsigned d = unsigned((k >> lo) & dmask);
+    if (top == p0) atomicAdd(&h[0][d], 1u);
+    if (!same && top == p1) atomicAdd(&h[1][d], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kBins; i += blockDim.x) {
+    if (h[0][i]) atomicAdd(&st->hist[i], (unsigned long long)h[0][i]);
+    if (!same && h[1][i]) atomicAdd(&st->hist[kBins + i], (unsigned long long)h[1][i]);
+  }
+}
+
+// one block of 256 threads: find the bucket of each rank, extend the
+// prefixes, clear the histograms for the next pass
+__global__ __launch_bounds__(256) void sel_scan_kernel(SelState* st, int hi, int lo) {
+  __shared__ unsigned long long part[256];
+  __shared__ int found_d;
+  __shared__ long long found_below;
+  const int t = threadIdx.x;
+  constexpr int per = kBins / 256;
+  const bool same = (st->prefix[0] == st->prefix[1]);
+  for (int r = 0; r < 2; ++r) {
+    const unsigned long long* hr = st->hist + ((same ? 0 : r) * kBins);
+    unsigned long long loc = 0;
+    for (int i = 0; i < per; ++i) loc += hr[t * per + i];
+    part[t] = loc;
+    __syncthreads();
+    // inclusive scan of part (Hillis-Steele)
+    for (int off = 1; off < 256; off <<= 1) {
+      unsigned long long v = (t >= off) ? part[t - off] : 0;
+      __syncthreads();
+      part[t] += v;
+      __syncthreads();
+    }
+    const long long k = st->k[r];
+    const unsigned long long before = (t == 0) ? 0 : part[t - 1];
+    if ((long long)before <= k && k < (long long)part[t]) {
+      unsigned long long acc = before;
+      for (int i = 0; i < per; ++i) {
+        const unsigned long long c = hr[t * per + i];
+        if ((long long)(acc + c) > k) {
+          found_d = t * per + i;
+          found_below = (long long)acc;
+          break;
+        }
+        acc += c;
+      }
+    }
+    __syncthreads();
+    if (t == 0) {
+      st->k[r] = k - found_below;
+      st->prefix[r] = (st->prefix[r] << (hi - lo)) | uint64_t(found_d);
+    }
+    __syncthreads();
+  }
+  for (int i = t; i < 2 * kBins; i += 256) st->hist[i] = 0;
+}
+
+template <typename T>
+__global__ void sel_finish_kernel(SelState* st, int deviation, double factor, double* out) {
+  if (threadIdx.x != 0) return;
+  const double a = double(value_from_key<T>(st->prefix[0]));
+  const double b = double(value_from_key<T>(st->prefix[1]));
+  const double med = 0.5 * (a + b);
+  if (!deviation) {
+    st->med = med;
+    if (out) *out = med;
+  } else {
+    st->mad = 1.4826 * med;
+    if (out) *out = (factor > 0.0) ? st->mad * factor : st->mad;
+  }
+}
+
+template <typename T>
+int select_median_t(const SelSrc& s, bool deviation, SelState* st, double factor, double* out,
+                    hipStream_t stream) {
+  const int key_bits = sizeof(T) * 8;
+  hipLaunchKernelGGL(sel_init_kernel, dim3(1), dim3(256), 0, stream, st, s.count);
+  const int64_t E = (s.mode == 2) ? s.n : s.count;
+  const int cus = std::max(1, device_info().num_cus);
+  const unsigned blocks = unsigned(std::max<int64_t>(1, std::min<int64_t>((E + 2047) / 2048, 4 * cus)));
+  for (int hi = key_bits; hi > 0;) {
+    const int lo = std::max(0, hi - kBits);
+    hipLaunchKernelGGL(sel_hist_kernel<T>, dim3(blocks), dim3(256), 0, stream, s, st, hi, lo,
+                       deviation ? 1 : 0);
+    hipLaunchKernelGGL(sel_scan_kernel, dim3(1), dim3(256), 0, stream, st, hi, lo);
+    hi = lo;
+  }
+  hipLaunchKernelGGL(sel_finish_kernel<T>, dim3(1), dim3(64), 0, stream, st,
+                     deviation ? 1 : 0, factor, out);
+  WH_HIP_CHECK(hipGetLastError());
+  return WH_OK;
+}
+
+int select_median(const SelSrc& s, int dtype, bool deviation, SelState* st, double factor,
+                  double* out, hipStream_t stream) {
+  if (s.count <= 0) return set_error(WH_ERR_ARG, "median of an empty selection");
+  return dtype == WH_F32 ? select_median_t<float>(s, deviation, st, factor, out, stream)
+                         : select_median_t<double>(s, deviation, st, factor, out, stream);
+}
+
+// MAD of the selection; result (times `factor` when > 0) to device `out`
+int select_mad(const SelSrc& s, int dtype, SelState* st, double factor, double* out,
+               hipStream_t stream) {
+  int rc = select_median(s, dtype, false, st, 0.0, nullptr, stream);
+  if (rc) return rc;
+  return select_median(s, dtype, true, st, factor, out, stream);
+}
+
+// --------------------------------------------------------- interp_mse
+template <typename T>
+__global__ __launch_bounds__(256) void interp_mse_partial(const T* __restrict__ xn,
+                                                          const T* __restrict__ ye,
+                                                          const T* __restrict__ yo, int64_t len,
+                                                          int64_t per_block, double* partial) {
+  __shared__ double red[256];
+  const int64_t lenh = len / 2;
+  const int64_t j0 = int64_t(blockIdx.x) * per_block;
+  const int64_t j1 = std::min(lenh, j0 + per_block);
+  double acc = 0.0;
+  for (int64_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
+    const int64_t i = 2 * j;
+    double yie, yio;
+    if (i == 0) {
+      yie = double(yo[0]);
+      yio = 0.5 * (double(ye[0]) + double(ye[1]));
+    } else if (i == len - 2) {
+      yio = double(ye[lenh - 1]);
+      yie = 0.5 * (double(yo[j]) + double(yo[j - 1]));
+    } else {
+      yio = 0.5 * (double(ye[j]) + double(ye[j + 1]));
+      yie = 0.5 * (double(yo[j]) + double(yo[j - 1]));
+    }
+    const double a = yie - double(xn[i]), b = yio - double(xn[i + 1]);
+    acc += a * a + b * b;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void sum_partials(const double* partial, int np, double scale,
+                                                    double* out) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) acc += partial[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0] * scale;
+}
+
+int interp_mse(const void* xn, const void* ye, const void* yo, int64_t len, int dtype,
+               double* partial_buf, double* out, hipStream_t st) {
+  const int64_t lenh = len / 2;
+  const int64_t per_block = 2048;
+  const int np = int(std::min<int64_t>((lenh + per_block - 1) / per_block, kMaxPartials));
+  const int64_t pb = (lenh + np - 1) / np;
+  if (dtype == WH_F32)
+    hipLaunchKernelGGL(interp_mse_partial<float>, dim3(np), dim3(256), 0, st,
+                       static_cast<const float*>(xn), static_cast<const float*>(ye),
+                       static_cast<const float*>(yo), len, pb, partial_buf);
+  else
+    hipLaunchKernelGGL(interp_mse_partial<double>, dim3(np), dim3(256), 0, st,
+                       static_cast<const double*>(xn), static_cast<const double*>(ye),
+                       static_cast<const double*>(yo), len, pb, partial_buf);
+  hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, st, partial_buf, np, 0.5, out);
+  WH_HIP_CHECK(hipGetLastError());
+  return WH_OK;
+}
+
+// ------------------------------------------------------------- helpers
+template <typename T>
+__global__ void split_even_odd(const T* __restrict__ x, T* __restrict__ e, T* __restrict__ o,
+                               int64_t lenh) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < lenh; j += stride) {
+    e[j] = x[2 * j];
+    o[j] = x[2 * j + 1];
+  }
+}
+
+int split_halves(const void* x, void* e, void* o, int64_t len, int dtype, hipStream_t st) {
+  const int64_t lenh = len / 2;
+  const unsigned blocks = unsigned(std::max<int64_t>(1, std::min<int64_t>((lenh + 255) / 256, 4096)));
+  if (dtype == WH_F32)
+    hipLaunchKernelGGL(split_even_odd<float>, dim3(blocks), dim3(256), 0, st,
+                       static_cast<const float*>(x), static_cast<float*>(e),
+                       static_cast<float*>(o), lenh);
+  else
+    hipLaunchKernelGGL(split_even_odd<double>, dim3(blocks), dim3(256), 0, st,
+                       static_cast<const double*>(x), static_cast<double*>(e),
+                       static_cast<double*>(o), lenh);
+  WH_HIP_CHECK(hipGetLastError());
+  return WH_OK;
+}
+
+__global__ void fill_levels(double* dst, const double* src, double scale, int J) {
+  const int t = threadIdx.x;
+  if (t < J) dst[t] = (*src) * scale;
+}
+
+int broadcast_threshold(double* dst, const double* src, double scale, int J, hipStream_t st) {
+  hipLaunchKernelGGL(fill_levels, dim3(1), dim3(64), 0, st, dst, src, scale, J);
+  WH_HIP_CHECK(hipGetLastError());
+  return WH_OK;
+}
+
+}  // namespace wh

@@ -1,0 +1,43 @@
+// wh_internal.h -- host-side internals shared by the wavehip source files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/wavehip.h"
+
+namespace wh {
+
+struct DeviceInfo {
+  int device = -1;
+  int num_cus = 0;
+  size_t max_lds_per_block = 0;
+  int warp_size = 64;
+  std::string arch;
+};
+
+// Info of the current device (queried once per device).
+const DeviceInfo& device_info();
+
+// Library-owned device scratch of at least `bytes`, per device, grown on
+// demand and kept until process exit.  Calls that overlap on one device
+// through different streams must not share it; the library's own entry
+// points order their use on the caller's stream.
+int scratch_get(size_t bytes, void** out);
+// A second, independent scratch region (used where two are live at once).
+int scratch2_get(size_t bytes, void** out);
+
+// transform back ends (device pointers)
+int modwt_forward(const void* x, void* xmod, int64_t n, int J, int filt, int dtype,
+                  hipStream_t st);
+int modwt_inverse(const void* xmod, void* x, int64_t n, int J, int filt, int dtype,
+                  const wh_thresh* thr, hipStream_t st);
+std::string modwt_plan_string(bool inverse, int64_t n, int J, int filt, int dtype);
+
+int dwt_forward(const void* x, void* y, int64_t n, int J, int filt, int dtype, hipStream_t st);
+int dwt_inverse(const void* y, void* x, int64_t n, int J, int filt, int dtype,
+                const wh_thresh* thr, hipStream_t st);
+
+}  // namespace wh

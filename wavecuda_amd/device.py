@@ -1,0 +1,148 @@
+"""Tensor-level API over the C ABI: torch provides device memory and the
+stream, libwavehip.so does all the arithmetic.
+
+Every function requires CUDA (HIP) tensors; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib as W
+
+_DT = {torch.float64: W.WH_F64, torch.float32: W.WH_F32}
+
+
+def filter_code(filter) -> int:
+    if isinstance(filter, str):
+        try:
+            return W.FILTER_CODES[filter]
+        except KeyError:
+            raise ValueError(f"unknown filter {filter!r}") from None
+    return int(filter)
+
+
+def _dtype(t: torch.Tensor) -> int:
+    if t.dtype not in _DT:
+        raise TypeError(f"unsupported dtype {t.dtype}; use float64 or float32")
+    return _DT[t.dtype]
+
+
+def _check_dev(*ts):
+    for t in ts:
+        if not t.is_cuda:
+            raise ValueError("wavecuda_amd needs tensors on the GPU (device='cuda')")
+        if not t.is_contiguous():
+            raise ValueError("tensors must be contiguous")
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def resolve_levels(n: int, nlevels: int, filter) -> int:
+    J = W.lib().wh_resolve_levels(int(n), int(nlevels), filter_code(filter))
+    if J <= 0:
+        raise ValueError(f"invalid length {n} / nlevels {nlevels} for filter {filter}")
+    return J
+
+
+def _thresh_struct(thresh):
+    if thresh is None:
+        return None, None
+    if isinstance(thresh, W.wh_thresh):
+        return thresh, None
+    value, hardness, minlevel, maxlevel, levels = (
+        thresh.get("value", 0.0),
+        thresh.get("hardness", W.SOFT),
+        thresh.get("minlevel", 0),
+        thresh.get("maxlevel", 0),
+        thresh.get("levels"),
+    )
+    keep = None
+    ptr = None
+    if levels is not None:
+        keep = levels.to(dtype=torch.float64).contiguous()
+        _check_dev(keep)
+        ptr = keep.data_ptr()
+    t = W.make_thresh(True, hardness, minlevel, maxlevel, value, ptr)
+    return t, keep
+
+
+def modwt_fwd(x: torch.Tensor, nlevels: int = 0, filter="LA8", out: torch.Tensor | None = None):
+    """Time-ordered MODWT: returns xmod of 2*n*J values [S1|D1|...|SJ|DJ]."""
+    _check_dev(x)
+    n = x.numel()
+    f = filter_code(filter)
+    J = resolve_levels(n, nlevels, f)
+    if out is None:
+        out = torch.empty(2 * n * J, dtype=x.dtype, device=x.device)
+    _check_dev(out)
+    W.check(W.lib().wh_modwt_fwd(x.data_ptr(), out.data_ptr(), n, J, f, _dtype(x), _stream()),
+            "wh_modwt_fwd")
+    return out
+
+
+def modwt_inv(xmod: torch.Tensor, n: int, nlevels: int = 0, filter="LA8", thresh=None,
+              out: torch.Tensor | None = None):
+    """Inverse MODWT of xmod (2*n*J values) with optional fused thresholding.
+
+    ``thresh`` is None or a dict(value=, hardness=, minlevel=, maxlevel=,
+    levels=<device tensor of per-level thresholds>)."""
+    _check_dev(xmod)
+    f = filter_code(filter)
+    J = resolve_levels(n, nlevels, f)
+    if xmod.numel() < 2 * n * J:
+        raise ValueError("xmod is shorter than 2*n*nlevels")
+    if out is None:
+        out = torch.empty(n, dtype=xmod.dtype, device=xmod.device)
+    _check_dev(out)
+    t, keep = _thresh_struct(thresh)
+    W.check(
+        W.lib().wh_modwt_inv(xmod.data_ptr(), out.data_ptr(), n, J, f, _dtype(xmod),
+                             C.byref(t) if t is not None else None, _stream()),
+        "wh_modwt_inv")
+    del keep
+    return out
+
+
+def dwt_fwd(x: torch.Tensor, nlevels: int = 0, filter="LA8", out: torch.Tensor | None = None):
+    """Periodic DWT in the interleaved layout (out may be x for in place)."""
+    _check_dev(x)
+    n = x.numel()
+    f = filter_code(filter)
+    J = resolve_levels(n, nlevels, f)
+    if out is None:
+        out = torch.empty_like(x)
+    _check_dev(out)
+    W.check(W.lib().wh_dwt_fwd(x.data_ptr(), out.data_ptr(), n, J, f, _dtype(x), _stream()),
+            "wh_dwt_fwd")
+    return out
+
+
+def dwt_inv(y: torch.Tensor, nlevels: int = 0, filter="LA8", thresh=None,
+            out: torch.Tensor | None = None):
+    """Inverse periodic DWT with optional fused thresholding."""
+    _check_dev(y)
+    n = y.numel()
+    f = filter_code(filter)
+    J = resolve_levels(n, nlevels, f)
+    if out is None:
+        out = torch.empty_like(y)
+    _check_dev(out)
+    t, keep = _thresh_struct(thresh)
+    W.check(
+        W.lib().wh_dwt_inv(y.data_ptr(), out.data_ptr(), n, J, f, _dtype(y),
+                           C.byref(t) if t is not None else None, _stream()),
+        "wh_dwt_inv")
+    del keep
+    return out
+
+
+def modwt_plan(inverse: bool, n: int, nlevels: int, filter="LA8", dtype=torch.float64) -> str:
+    buf = C.create_string_buffer(4096)
+    rc = W.lib().wh_debug_modwt_plan(1 if inverse else 0, n, nlevels, filter_code(filter),
+                                    _DT[dtype], buf, 4096)
+    W.check(rc, "wh_debug_modwt_plan")
+    return buf.value.decode()
