@@ -1,0 +1,197 @@
+"""Benchmark of the wavelet hot path on one MI355X.
+
+Step = one LA8 time-ordered MODWT with J = 10 levels, forward then inverse,
+over one n = 2^24 FP64 signal already resident in HBM (white noise).  The
+headline value is whole-job throughput in Msamples/s.  Alongside:
+
+  roofline      the dominant kernel (the fused forward MODWT) priced against
+                MI355X's 8 TB/s HBM: algorithmic bytes per launch
+                (8 (1 + 2J) B per sample: read x once, write S_j and D_j for
+                every level) / its average duration, timed with HIP events
+                on the stream the kernel runs on;
+  cpu_baseline  the CPU oracle (the reference's serial CPU path restated)
+                timed on this host on a bounded sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "LA8 DWT/MODWT fwd+inv Msamples/s & %HBM peak; batched vs plain at same tol"
+
+
+def modwt_bytes_fwd(n, J, esz=8):
+    return esz * n * (1 + 2 * J)
+
+
+def modwt_bytes_inv(n, J, esz=8):
+    return esz * n * (J + 2)
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the forward kernel measured by rocprofv3
+    FETCH_SIZE / WRITE_SIZE (scaled by a copy-kernel calibration), if the
+    committed summary exists."""
+    p = ROOT / "profiles" / "modwt_fwd_la8_n2p24_J10_traffic.txt"
+    if not p.exists():
+        return None
+    m = re.search(r"^hbm_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(n_sample=1 << 22, J=10, filt=4):
+    """Single-threaded oracle LA8 MODWT fwd+inv on n_sample samples."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import numpy as np
+
+    import oracle_bind as O  # test infrastructure: the checker, timed as a baseline
+
+    x = np.random.Generator(np.random.PCG64(1)).standard_normal(n_sample)
+    O.modwt_fwd(x[:1024], J if J <= 7 else 7, filt)  # load / warm
+    reps, t_total = 0, 0.0
+    while t_total < 10.0 and reps < 4:
+        t0 = time.perf_counter()
+        xm = O.modwt_fwd(x, J, filt)
+        xr = O.modwt_inv(xm, n_sample, J, filt)
+        t_total += time.perf_counter() - t0
+        reps += 1
+        del xm, xr
+    rate = reps * n_sample / t_total / 1e6
+    return {
+        "value": round(rate, 3),
+        "unit": "Msamples/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"LA8 MODWT J={J} fwd+inv on n=2^{n_sample.bit_length() - 1} FP64, "
+                  f"{reps} reps, single thread, oracle/wvoracle.c",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--log2n", type=int, default=24)
+    ap.add_argument("--levels", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from wavecuda_amd import _lib as W
+    from wavecuda_amd import device as D
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    W.check(W.lib().wh_init(0), "wh_init")
+    n = 1 << args.log2n
+    J = args.levels
+    filt = W.LA8
+    g = torch.Generator(device="cuda").manual_seed(1234)
+    x = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    xmod = torch.empty(2 * n * J, dtype=torch.float64, device="cuda")
+    xr = torch.empty(n, dtype=torch.float64, device="cuda")
+
+    def step():
+        D.modwt_fwd(x, J, filt, out=xmod)
+        D.modwt_inv(xmod, n, J, filt, out=xr)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    err = ((xr - x).abs().max() / x.abs().max()).item()
+    if not err <= 1e-12:
+        print(f"reconstruction error {err} exceeds 1e-12", file=sys.stderr)
+        sys.exit(1)
+
+    # timed region: K steps, events around each kernel on the current stream
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1, e2 in ev:
+        e0.record()
+        D.modwt_fwd(x, J, filt, out=xmod)
+        e1.record()
+        D.modwt_inv(xmod, n, J, filt, out=xr)
+        e2.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    fwd_ms = sum(a.elapsed_time(b) for a, b, _ in ev) / args.steps
+    inv_ms = sum(b.elapsed_time(c) for _, b, c in ev) / args.steps
+    ms_per_step = wall * 1e3 / args.steps
+
+    value = n / (ms_per_step * 1e-3) / 1e6
+    bf, bi = modwt_bytes_fwd(n, J), modwt_bytes_inv(n, J)
+    fwd_gbs = bf / (fwd_ms * 1e-3) / 1e9
+    inv_gbs = bi / (inv_ms * 1e-3) / 1e9
+    step_gbs = (bf + bi) / (ms_per_step * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"LA8 MODWT J={J} forward+inverse, one signal of n=2^{args.log2n}, "
+                        "FP64 white noise resident in HBM",
+            "transform": "MODWT (time ordered)",
+            "filter": "LA8",
+            "n": n,
+            "levels": J,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "modwt_fwd_group<double, LA8>",
+            "achieved": round(fwd_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(fwd_gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles(),
+            "algorithmic_bytes_per_launch": bf,
+            "avg_launch_ms": round(fwd_ms, 4),
+        },
+        "inverse_kernel": {
+            "kernel": "modwt_inv_group<double, LA8>",
+            "achieved_GBs": round(inv_gbs, 1),
+            "frac": round(inv_gbs / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes_per_launch": bi,
+            "avg_launch_ms": round(inv_ms, 4),
+        },
+        "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+        "reconstruction_relerr": err,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(J=J)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,40 @@
+"""The C-ABI library builds, loads without a GPU and exports every function
+declared in include/wavehip.h (no compute calls here)."""
+import re
+from pathlib import Path
+
+import pytest
+
+from wavecuda_amd import _lib as W
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_functions():
+    text = (ROOT / "include" / "wavehip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\([^;{]*\)\s*;", text)
+    return sorted({n for n in names if n not in {"if", "while", "for", "return", "sizeof"}})
+
+
+def test_header_lists_entry_points():
+    names = declared_functions()
+    for must in ("RgpuTransform", "RgpuSmooth", "wh_modwt_fwd", "wh_modwt_inv", "wh_dwt_fwd",
+                 "wh_dwt_inv", "wh_smooth", "wh_mad", "wh_univ_thresh"):
+        assert must in names
+
+
+@pytest.mark.parametrize("name", declared_functions())
+def test_symbol_exported(name):
+    assert W.has(name), f"{name} declared in include/wavehip.h but not exported"
+
+
+def test_no_device_needed_for_pure_queries():
+    L = W.lib()
+    assert L.wh_version() >= 100
+    assert L.wh_resolve_levels(1024, 0, 2) == 9
+    assert L.wh_resolve_levels(1024, 0, 4) == 8
+    assert L.wh_resolve_levels(1000, 0, 1) == 0
+    assert L.wh_resolve_levels(256, 7, 4) == 0
+    assert L.wh_resolve_levels(8, 0, 3) == 1
+    assert isinstance(L.wh_last_error(), bytes)

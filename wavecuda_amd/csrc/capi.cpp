@@ -11,6 +11,7 @@
 
 #include "wh_common.h"
 #include "wh_internal.h"
+#include "wh_stats.h"
 
 namespace wh {
 
@@ -24,7 +25,7 @@ struct Scratch {
   void* p = nullptr;
   size_t bytes = 0;
 };
-Scratch g_scratch[kMaxDev][2];
+Scratch g_scratch[kMaxDev][4];
 
 int current_device() {
   int d = -1;
@@ -90,6 +91,8 @@ const DeviceInfo& device_info() {
 
 int scratch_get(size_t bytes, void** out) { return scratch_impl(0, bytes, out); }
 int scratch2_get(size_t bytes, void** out) { return scratch_impl(1, bytes, out); }
+int ws_get(size_t bytes, void** out) { return scratch_impl(2, bytes, out); }
+int stats_get(size_t bytes, void** out) { return scratch_impl(3, bytes, out); }
 
 }  // namespace wh
 
@@ -177,6 +180,83 @@ int wh_dwt_inv(const void* y, void* x, int64_t n, int nlevels, int filter, int d
   return dwt_inverse(y, x, n, J, filter, dtype, thr, static_cast<hipStream_t>(stream));
 }
 
+int wh_threshold(void* coeffs, int64_t n, int nlevels, int ttype, int filter, int dtype,
+                 const wh_thresh* thr, void* stream) {
+  int J, rc = check_common(coeffs, coeffs, n, nlevels, filter, dtype, &J);
+  if (rc) return rc;
+  if (ttype != kDWT && ttype != kMODWT) return set_error(WH_ERR_ARG, "ttype must be 0 or 1");
+  if (!thr || !thr->enabled) return WH_OK;
+  if ((rc = check_thresh(thr, J))) return rc;
+  return threshold_inplace(coeffs, n, J, ttype, dtype, thr, static_cast<hipStream_t>(stream));
+}
+
+static int stats_state(SelState** st) {
+  void* p = nullptr;
+  int rc = stats_get(sizeof(SelState) + (kMaxPartials + 80) * sizeof(double), &p);
+  if (rc) return rc;
+  *st = static_cast<SelState*>(p);
+  return WH_OK;
+}
+
+int wh_median(const void* x, int64_t count, int dtype, double* out, void* stream) {
+  if (!x || !out || count <= 0) return set_error(WH_ERR_ARG, "null pointer or empty input");
+  if (!valid_dtype(dtype)) return set_error(WH_ERR_ARG, "bad dtype");
+  SelState* st;
+  int rc = stats_state(&st);
+  if (rc) return rc;
+  SelSrc s{};
+  s.base = x;
+  s.mode = 0;
+  s.count = count;
+  return select_median(s, dtype, false, st, 0.0, out, static_cast<hipStream_t>(stream));
+}
+
+int wh_mad(const void* x, int64_t count, int dtype, double* out, void* stream) {
+  if (!x || !out || count <= 0) return set_error(WH_ERR_ARG, "null pointer or empty input");
+  if (!valid_dtype(dtype)) return set_error(WH_ERR_ARG, "bad dtype");
+  SelState* st;
+  int rc = stats_state(&st);
+  if (rc) return rc;
+  SelSrc s{};
+  s.base = x;
+  s.mode = 0;
+  s.count = count;
+  return select_mad(s, dtype, st, 0.0, out, static_cast<hipStream_t>(stream));
+}
+
+int wh_univ_thresh(const void* coeffs, int64_t n, int nlevels, int ttype, int filter, int dtype,
+                   int minlevel, int maxlevel, double* out, void* stream) {
+  int J, rc = check_common(coeffs, out, n, nlevels, filter, dtype, &J);
+  if (rc) return rc;
+  if (ttype != kDWT && ttype != kMODWT) return set_error(WH_ERR_ARG, "ttype must be 0 or 1");
+  if (minlevel < 0 || minlevel > maxlevel || maxlevel >= J)
+    return set_error(WH_ERR_ARG, "levels must satisfy 0 <= minlevel <= maxlevel < nlevels");
+  return univ_thresh_dev(coeffs, n, J, ttype, dtype, minlevel, maxlevel, out,
+                         static_cast<hipStream_t>(stream));
+}
+
+int wh_interp_mse(const void* xn, const void* ye, const void* yo, int64_t n, int dtype,
+                  double* out, void* stream) {
+  if (!xn || !ye || !yo || !out) return set_error(WH_ERR_ARG, "null pointer");
+  if (n < 4 || (n & 1)) return set_error(WH_ERR_ARG, "length must be even and >= 4");
+  if (!valid_dtype(dtype)) return set_error(WH_ERR_ARG, "bad dtype");
+  void* p = nullptr;
+  int rc = stats_get(sizeof(SelState) + (kMaxPartials + 80) * sizeof(double), &p);
+  if (rc) return rc;
+  double* partial = reinterpret_cast<double*>(static_cast<char*>(p) + sizeof(SelState));
+  return interp_mse(xn, ye, yo, n, dtype, partial, out, static_cast<hipStream_t>(stream));
+}
+
+int wh_smooth(void* x, int64_t n, int nlevels, int ttype, int filter, int dtype, int threshtype,
+              double* thresh, int hardness, int minlevel, int maxlevel, double tol,
+              void* stream) {
+  int J, rc = check_common(x, x, n, nlevels, filter, dtype, &J);
+  if (rc) return rc;
+  if (ttype != kDWT && ttype != kMODWT) return set_error(WH_ERR_ARG, "ttype must be 0 or 1");
+  return smooth(x, n, J, ttype, filter, dtype, threshtype, thresh, hardness, minlevel, maxlevel,
+                tol, static_cast<hipStream_t>(stream));
+}
+
 // Diagnostic: the level-group plan chosen for a MODWT (written to buf).
 int wh_debug_modwt_plan(int inverse, int64_t n, int nlevels, int filter, int dtype, char* buf,
                         int cap) {
@@ -256,6 +336,33 @@ void RgpuTransform(double* x, double* xmod, int* len, int* sense, int* nlevels, 
     return;
   }
   report("transform", set_error(WH_ERR_ARG, "ttype must be 0 (DWT) or 1 (MODWT)"));
+}
+
+void RgpuSmooth(double* x, int* len, int* nlevels, int* ttype, int* filter, int* filterlen,
+                int* threshtype, double* thresh, int* hardness, int* minlevel, int* maxlevel,
+                double* tol) {
+  (void)filterlen;
+  if (!x || !len || !nlevels || !ttype || !filter || !threshtype || !hardness || !minlevel ||
+      !maxlevel) {
+    report("smooth", set_error(WH_ERR_ARG, "null argument"));
+    return;
+  }
+  const int64_t n = *len;
+  const int J = valid_filter(*filter) ? resolve_levels(n, *nlevels, filter_length(*filter)) : 0;
+  if (J <= 0) return report("smooth", set_error(WH_ERR_ARG, "invalid length, levels or filter"));
+  DevBuf a;
+  int rc;
+  if ((rc = a.alloc(n * 8))) return report("smooth", rc);
+  if (hipMemcpy(a.p, x, n * 8, hipMemcpyHostToDevice) != hipSuccess)
+    return report("smooth", set_error(WH_ERR_HIP, "copy in"));
+  double t = thresh ? *thresh : 0.0;
+  const double tl = tol ? *tol : 0.01;
+  if ((rc = wh_smooth(a.p, n, J, *ttype, *filter, WH_F64, *threshtype, &t, *hardness, *minlevel,
+                      *maxlevel, tl, nullptr)))
+    return report("smooth", rc);
+  if (hipMemcpy(x, a.p, n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    return report("smooth", set_error(WH_ERR_HIP, "copy out"));
+  if (thresh && *threshtype != 0) *thresh = t;
 }
 
 }  // extern "C"

@@ -15,6 +15,7 @@
 //    rounds like the reference;
 //  * the CV interpolation error (reference src/thresh.cpp:344-384) as a
 //    fixed-order two-stage reduction, so repeated runs are bit-identical.
+#include <algorithm>
 #include <cmath>
 
 #include "wh_common.h"

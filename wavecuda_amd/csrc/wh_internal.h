@@ -26,8 +26,12 @@ const DeviceInfo& device_info();
 // through different streams must not share it; the library's own entry
 // points order their use on the caller's stream.
 int scratch_get(size_t bytes, void** out);
-// A second, independent scratch region (used where two are live at once).
+// Independent regions for buffers that are live at the same time:
+// scratch2 (in-place staging of the DWT), ws (pipeline workspace of the
+// smoothing / batched paths) and stats (small selection/reduction state).
 int scratch2_get(size_t bytes, void** out);
+int ws_get(size_t bytes, void** out);
+int stats_get(size_t bytes, void** out);
 
 // transform back ends (device pointers)
 int modwt_forward(const void* x, void* xmod, int64_t n, int J, int filt, int dtype,
@@ -39,5 +43,11 @@ std::string modwt_plan_string(bool inverse, int64_t n, int J, int filt, int dtyp
 int dwt_forward(const void* x, void* y, int64_t n, int J, int filt, int dtype, hipStream_t st);
 int dwt_inverse(const void* y, void* x, int64_t n, int J, int filt, int dtype,
                 const wh_thresh* thr, hipStream_t st);
+
+int univ_thresh_dev(const void* c, int64_t n, int J, int ttype, int dtype, int minlevel,
+                    int maxlevel, double* out, hipStream_t st);
+int smooth(void* x, int64_t n, int J, int ttype, int filt, int dtype, int threshtype,
+           double* thresh, int hardness, int minlevel, int maxlevel, double tol,
+           hipStream_t st);
 
 }  // namespace wh

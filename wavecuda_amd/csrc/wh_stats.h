@@ -1,0 +1,55 @@
+// wh_stats.h -- thresholding / statistics back ends shared by stats.hip and
+// smooth.cpp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/wavehip.h"
+
+namespace wh {
+
+// per-level (or single) threshold handed to the in-place kernels
+struct Thr1 {
+  int hardness, minlevel, maxlevel;
+  double value;
+  const double* levels;  // device, one per level, or nullptr
+};
+
+// Values taking part in a median:
+//   mode 0: base[off .. off+count)
+//   mode 1: MODWT detail blocks 0..(count/n - 1): base[2 n b + n + i]
+//   mode 2: DWT details of 0-based levels [lmin, lmax] (positions with
+//           ctz in range and < J) of an n-long interleaved vector
+struct SelSrc {
+  const void* base;
+  int mode;
+  int64_t n;
+  int64_t count;
+  int64_t off;
+  int lmin, lmax, J;
+};
+
+// device-resident selection state
+struct SelState {
+  uint64_t prefix[2];
+  long long k[2];
+  double med;
+  double mad;
+  unsigned long long hist[2 * 2048];
+};
+
+constexpr int kMaxPartials = 1024;
+
+int threshold_inplace(void* c, int64_t n, int J, int ttype, int dtype, const wh_thresh* thr,
+                      hipStream_t st);
+int select_median(const SelSrc& s, int dtype, bool deviation, SelState* st, double factor,
+                  double* out, hipStream_t stream);
+int select_mad(const SelSrc& s, int dtype, SelState* st, double factor, double* out,
+               hipStream_t stream);
+int interp_mse(const void* xn, const void* ye, const void* yo, int64_t len, int dtype,
+               double* partial_buf, double* out, hipStream_t st);
+int split_halves(const void* x, void* e, void* o, int64_t len, int dtype, hipStream_t st);
+int broadcast_threshold(double* dst, const double* src, double scale, int J, hipStream_t st);
+
+}  // namespace wh

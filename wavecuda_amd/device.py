@@ -146,3 +146,70 @@ def modwt_plan(inverse: bool, n: int, nlevels: int, filter="LA8", dtype=torch.fl
                                     _DT[dtype], buf, 4096)
     W.check(rc, "wh_debug_modwt_plan")
     return buf.value.decode()
+
+
+def threshold_(coeffs: torch.Tensor, n: int, nlevels: int, ttype: int, filter, value: float,
+               hardness: int, minlevel: int, maxlevel: int, levels: torch.Tensor | None = None):
+    """In-place thresholding of a transformed vector (DWT n / MODWT 2nJ values)."""
+    _check_dev(coeffs)
+    f = filter_code(filter)
+    J = resolve_levels(n, nlevels, f)
+    t, keep = _thresh_struct(dict(value=value, hardness=hardness, minlevel=minlevel,
+                                  maxlevel=maxlevel, levels=levels))
+    W.check(W.lib().wh_threshold(coeffs.data_ptr(), n, J, int(ttype), f, _dtype(coeffs),
+                                 C.byref(t), _stream()), "wh_threshold")
+    del keep
+    return coeffs
+
+
+def _scalar_out(fn, what, *args):
+    out = torch.empty(1, dtype=torch.float64, device="cuda")
+    W.check(fn(*args, out.data_ptr(), _stream()), what)
+    return out
+
+
+def median(x: torch.Tensor) -> torch.Tensor:
+    """Exact median (average of the two middle order statistics), 1 double."""
+    _check_dev(x)
+    return _scalar_out(W.lib().wh_median, "wh_median", x.data_ptr(), x.numel(), _dtype(x))
+
+
+def mad(x: torch.Tensor) -> torch.Tensor:
+    """1.4826 * median(|x - median(x)|), 1 double."""
+    _check_dev(x)
+    return _scalar_out(W.lib().wh_mad, "wh_mad", x.data_ptr(), x.numel(), _dtype(x))
+
+
+def univ_thresh(coeffs: torch.Tensor, n: int, nlevels: int, ttype: int, filter,
+                minlevel: int = 0, maxlevel: int = 0) -> torch.Tensor:
+    _check_dev(coeffs)
+    f = filter_code(filter)
+    J = resolve_levels(n, nlevels, f)
+    return _scalar_out(W.lib().wh_univ_thresh, "wh_univ_thresh", coeffs.data_ptr(), n, J,
+                       int(ttype), f, _dtype(coeffs), int(minlevel), int(maxlevel))
+
+
+def interp_mse(xn: torch.Tensor, ye: torch.Tensor, yo: torch.Tensor) -> torch.Tensor:
+    _check_dev(xn, ye, yo)
+    return _scalar_out(W.lib().wh_interp_mse, "wh_interp_mse", xn.data_ptr(), ye.data_ptr(),
+                       yo.data_ptr(), xn.numel(), _dtype(xn))
+
+
+def smooth(x: torch.Tensor, nlevels: int, ttype: int, filter, threshtype: int,
+           thresh: float | None = None, hardness: int = 1, minlevel: int = 0,
+           maxlevel: int | None = None, tol: float = 0.01, inplace: bool = False):
+    """Forward -> threshold (0 manual / 1 universal / 2 CV) -> inverse.
+
+    Returns (smoothed tensor, threshold used)."""
+    _check_dev(x)
+    f = filter_code(filter)
+    n = x.numel()
+    J = resolve_levels(n, nlevels, f)
+    if maxlevel is None:
+        maxlevel = J - 1
+    y = x if inplace else x.clone()
+    t = C.c_double(0.0 if thresh is None else float(thresh))
+    W.check(W.lib().wh_smooth(y.data_ptr(), n, J, int(ttype), f, _dtype(y), int(threshtype),
+                              C.byref(t), int(hardness), int(minlevel), int(maxlevel),
+                              float(tol), _stream()), "wh_smooth")
+    return y, t.value
