@@ -186,6 +186,7 @@ def main():
             "avg_launch_ms": round(inv_ms, 4),
         },
         "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+        "plan": {"fwd": D.modwt_plan(False, n, J, filt), "inv": D.modwt_plan(True, n, J, filt)},
         "reconstruction_relerr": err,
     }
     if not args.no_cpu:

@@ -9,21 +9,25 @@
 // Kernel design.  A "group" fuses g consecutive levels j0+1..j0+g.  Level
 // j0+l only couples positions congruent modulo B = 2^{j0}, so position
 // i = r + B m is split into a residue r and an index m; inside the group
-// level l has stride 2^{l-1} in m.  A workgroup owns R consecutive residues
-// (R = min(B, 32), giving 8R-byte contiguous HBM runs) and one segment
-// [mlo, mhi) of m, and marches through the segment in chunks of M
+// level l has stride 2^{l-1} in m.  A workgroup owns R residues (R = 1 for
+// the group that starts at level 1, R = 32 otherwise: 256-byte HBM runs)
+// and one segment [mlo, mhi) of m, and marches through it in chunks of M
 // (descending for the forward, whose taps look ahead; ascending for the
-// inverse, whose taps look back).  Each level keeps an LDS ring holding the
-// chunk plus the (L-1) 2^{l-1} window values left over from the previous
-// chunk, so every level of the group reads its input from LDS and HBM sees
-// each group input once and each output once.  The segment is entered
-// after a warm-up of (L-1)(2^g - 1) m-positions (rounded up to M) whose
-// outputs are computed but not stored -- the only redundant work.
-// LDS layout is [slot][residue] so both the stride-2^{l-1} taps and the
-// HBM runs are contiguous across lanes (no bank conflicts for R = 1 or 32).
-// Loads of the next chunk are issued before a level's arithmetic and
-// written to LDS after the level's barrier, hiding HBM latency behind
-// compute with one workgroup per CU.
+// inverse, whose taps look back).  Every level keeps an LDS ring with the
+// chunk plus the (L-1) 2^{l-1} window values left from earlier chunks, so
+// all levels of a group read their input from LDS and HBM sees each group
+// input once and each output once.  A segment is entered after a warm-up of
+// (L-1)(2^g - 1) positions (rounded up to M) whose outputs are computed but
+// not stored -- the only redundant work.
+//
+// Each thread owns two adjacent elements (consecutive m when R = 1,
+// consecutive residues otherwise), so LDS reads/writes and HBM loads/stores
+// are 16-byte vectors; ring slots are kept even so a pair never straddles
+// the ring's wrap.  Levels are unrolled with compile-time strides, the next
+// chunk's HBM loads are issued before a chunk's arithmetic and parked in
+// LDS after it, and the LDS layout [slot][residue] keeps lanes contiguous.
+// The threshold of GPUSmooth is applied to the detail blocks as the
+// inverse loads them.
 #include <algorithm>
 #include <mutex>
 #include <vector>
@@ -40,7 +44,7 @@ struct ModwtArgs {
   void* out;             // inv: S_{j0} destination (x when j0 == 0)
   int64_t n;
   int64_t P;             // segment length in m units
-  int j0, g, logR, M;
+  int j0, g, M;
   int warm;              // warm-up in m units, multiple of M
   int nchunks;           // (P + warm) / M
   // fused threshold (inverse only)
@@ -49,271 +53,461 @@ struct ModwtArgs {
   const double* th_levels;
 };
 
+constexpr int kGmax = 10;
+
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+
+__host__ __device__ constexpr int even_up(int v) { return (v + 1) & ~1; }
+
+// Ring geometry in m units.  Forward ring l (l = 0..g-1) holds the input of
+// level l+1, window (L-1) 2^l; inverse ring l (l = 1..g) holds S_l and D_l,
+// window (L-1) 2^{l-1}.  Windows are rounded up to even.
+__host__ __device__ constexpr int fwd_win(int L, int l) { return even_up((L - 1) << l); }
+__host__ __device__ constexpr int inv_win(int L, int l) { return even_up((L - 1) << (l - 1)); }
+__host__ __device__ inline int fwd_off(int L, int M, int l) {
+  int o = 0;
+  for (int i = 0; i < l; ++i) o += M + fwd_win(L, i);
+  return o;
+}
+__host__ __device__ inline int inv_off(int L, int M, int l) {
+  int o = 0;
+  for (int i = 1; i < l; ++i) o += M + inv_win(L, i);
+  return o;
+}
+
+template <typename T>
+__device__ __forceinline__ typename Vec2<T>::type ldv(const T* p) {
+  return *reinterpret_cast<const typename Vec2<T>::type*>(p);
+}
+template <typename T>
+__device__ __forceinline__ void stv(T* p, typename Vec2<T>::type v) {
+  *reinterpret_cast<typename Vec2<T>::type*>(p) = v;
+}
+template <typename T>
+__device__ __forceinline__ void stv_nt(T* p, typename Vec2<T>::type v) {
+  __builtin_nontemporal_store(v.x, p);
+  __builtin_nontemporal_store(v.y, p + 1);
+}
+
 // ---------------------------------------------------------------- forward
-template <typename T, int FILT, int NT, int ITEMS>
-__global__ __launch_bounds__(NT) void modwt_fwd_group(ModwtArgs a) {
+template <typename T, int FILT, int R, int NT>
+struct FwdCtx {
+  using V = typename Vec2<T>::type;
+  static constexpr int L = Filter<FILT>::L;
+  T* lds;
+  T* xmod;
+  int M, mm, r, g, j0;
+  bool active;
+  int64_t n, r0, mlo, mhi, c;
+  int head[kGmax];  // ring heads (slot of position c), ring l = 0..g-1
+
+  // level l (1-based within the group): ring l-1 -> S_{j0+l}, D_{j0+l}
+  template <int l>
+  __device__ __forceinline__ void level() {
+    constexpr int s = 1 << (l - 1);
+    const int Qin = M + fwd_win(L, l - 1);
+    const T* rin = lds + size_t(fwd_off(L, M, l - 1)) * R;
+    if (active) {
+      T s0, d0, s1, d1;
+      if constexpr (R == 1 && s == 1) {
+        // taps m..m+L-1 and m+1..m+L: L/2+1 aligned pairs from slot m
+        T w[L + 2];
+        int sl = head[l - 1] + mm;
+        if (sl >= Qin) sl -= Qin;
+#pragma unroll
+        for (int q = 0; q <= L / 2; ++q) {
+          int sq = sl + 2 * q;
+          if (sq >= Qin) sq -= Qin;
+          V v = ldv(rin + sq);
+          w[2 * q] = v.x;
+          w[2 * q + 1] = v.y;
+        }
+        if constexpr (FILT == kHAAR) {
+          s0 = (w[0] + w[1]) * T(kInvR2);
+          d0 = (w[0] - w[1]) * T(kInvR2);
+          s1 = (w[1] + w[2]) * T(kInvR2);
+          d1 = (w[1] - w[2]) * T(kInvR2);
+        } else {
+          s0 = w[0] * T(Filter<FILT>::c(0));
+          d0 = w[0] * T(gcoef<FILT>(0));
+          s1 = w[1] * T(Filter<FILT>::c(0));
+          d1 = w[1] * T(gcoef<FILT>(0));
+#pragma unroll
+          for (int k = 1; k < L; ++k) {
+            s0 = fma(w[k], T(Filter<FILT>::c(k)), s0);
+            d0 = fma(w[k], T(gcoef<FILT>(k)), d0);
+            s1 = fma(w[k + 1], T(Filter<FILT>::c(k)), s1);
+            d1 = fma(w[k + 1], T(gcoef<FILT>(k)), d1);
+          }
+        }
+      } else {
+        V v[L];
+        int sl = head[l - 1] + mm;
+        if (sl >= Qin) sl -= Qin;
+#pragma unroll
+        for (int k = 0; k < L; ++k) {
+          int sk = sl + k * s;
+          if (sk >= Qin) sk -= Qin;
+          v[k] = ldv(rin + size_t(sk) * R + r);
+        }
+        if constexpr (FILT == kHAAR) {
+          s0 = (v[0].x + v[1].x) * T(kInvR2);
+          d0 = (v[0].x - v[1].x) * T(kInvR2);
+          s1 = (v[0].y + v[1].y) * T(kInvR2);
+          d1 = (v[0].y - v[1].y) * T(kInvR2);
+        } else {
+          s0 = v[0].x * T(Filter<FILT>::c(0));
+          d0 = v[0].x * T(gcoef<FILT>(0));
+          s1 = v[0].y * T(Filter<FILT>::c(0));
+          d1 = v[0].y * T(gcoef<FILT>(0));
+#pragma unroll
+          for (int k = 1; k < L; ++k) {
+            s0 = fma(v[k].x, T(Filter<FILT>::c(k)), s0);
+            d0 = fma(v[k].x, T(gcoef<FILT>(k)), d0);
+            s1 = fma(v[k].y, T(Filter<FILT>::c(k)), s1);
+            d1 = fma(v[k].y, T(gcoef<FILT>(k)), d1);
+          }
+        }
+      }
+      if (l < g) {
+        const int Qo = M + fwd_win(L, l);
+        int so = head[l] + mm;
+        if (so >= Qo) so -= Qo;
+        stv(lds + size_t(fwd_off(L, M, l)) * R + size_t(so) * R + r, V{s0, s1});
+      }
+      const int64_t m = c + mm;
+      if (m >= mlo && m < mhi) {
+        const int64_t i = (R == 1) ? m : r0 + r + (m << j0);
+        T* oS = xmod + 2 * n * int64_t(j0 + l - 1) + i;
+        stv_nt(oS, V{s0, s1});
+        stv_nt(oS + n, V{d0, d1});
+      }
+    }
+    __syncthreads();
+  }
+
+  template <int l>
+  __device__ __forceinline__ void levels_from() {
+    if constexpr (l <= kGmax) {
+      if (l <= g) {
+        level<l>();
+        levels_from<l + 1>();
+      }
+    }
+  }
+};
+
+template <typename T, int FILT, int R, int NT>
+__global__ __launch_bounds__(NT) void modwt_fwd_v2(ModwtArgs a) {
+  using V = typename Vec2<T>::type;
   constexpr int L = Filter<FILT>::L;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* lds = reinterpret_cast<T*>(smem_raw);
-
-  const int tid = threadIdx.x;
-  const int logR = a.logR, R = 1 << logR, M = a.M, MR = M << logR;
-  const int64_t n = a.n, Nm = n >> a.j0;
-  const int64_t nrb = (int64_t(1) << a.j0) >> logR;
+  FwdCtx<T, FILT, R, NT> C;
+  C.lds = reinterpret_cast<T*>(smem_raw);
+  C.xmod = static_cast<T*>(a.xmod);
+  C.M = a.M;
+  C.g = a.g;
+  C.j0 = a.j0;
+  C.n = a.n;
+  const int e = 2 * int(threadIdx.x);
+  C.active = e < a.M * R;
+  C.mm = e / R;
+  C.r = e % R;
+  const int64_t Nm = a.n >> a.j0;
+  const int64_t nrb = (int64_t(1) << a.j0) / R;
   const int64_t rb = int64_t(blockIdx.x) % nrb, seg = int64_t(blockIdx.x) / nrb;
-  const int64_t r0 = rb << logR;
-  const int64_t mlo = seg * a.P, mhi = mlo + a.P;
-  const int g = a.g, j0 = a.j0;
+  C.r0 = rb * R;
+  C.mlo = seg * a.P;
+  C.mhi = C.mlo + a.P;
   const T* __restrict__ in = static_cast<const T*>(a.in);
-  T* __restrict__ xmod = static_cast<T*>(a.xmod);
-
-  // first chunk start (descending march)
-  const int64_t ctop = mhi + a.warm - M;
-
-  // prologue: chunk 0 into ring 0 (head 0)
+  const int64_t ctop = C.mhi + a.warm - a.M;
 #pragma unroll
-  for (int it = 0; it < ITEMS; ++it) {
-    int q = tid + it * NT;
-    if (q >= MR) break;
-    int mm = q >> logR, r = q & (R - 1);
-    int64_t m = pmod(ctop + mm, Nm);
-    lds[mm * R + r] = in[r0 + r + (m << j0)];
-  }
+  for (int l = 0; l < kGmax; ++l) C.head[l] = 0;
+
+  auto in_index = [&](int64_t m) -> int64_t {
+    m &= (Nm - 1);
+    return (R == 1) ? m : C.r0 + C.r + (m << a.j0);
+  };
+  // prologue: chunk 0 into ring 0
+  if (C.active) stv(C.lds + size_t(C.mm) * R + C.r, ldv(in + in_index(ctop + C.mm)));
   __syncthreads();
 
   for (int k = 0; k < a.nchunks; ++k) {
-    const int64_t c = ctop - int64_t(k) * M;
+    C.c = ctop - int64_t(k) * a.M;
     const bool have_next = (k + 1 < a.nchunks);
-    T pre[ITEMS];
-    for (int l = 1; l <= g; ++l) {
-      const int s = 1 << (l - 1);
-      const int Qin = M + (L - 1) * s;
-      const int offin = ((l - 1) * M + (L - 1) * (s - 1)) * R;
-      const int headin = (Qin - int((int64_t(k) * M) % Qin)) % Qin;
-      const int Qout = M + (L - 1) * 2 * s;
-      const int offout = (l * M + (L - 1) * (2 * s - 1)) * R;
-      const int headout = (Qout - int((int64_t(k) * M) % Qout)) % Qout;
-      const bool store = (l < g);
-      T* __restrict__ oS = xmod + 2 * n * int64_t(j0 + l - 1);
-      T* __restrict__ oD = oS + n;
-
-      if (l == 1 && have_next) {
-        const int64_t cn = c - M;
+    V pre{};
+    if (have_next && C.active) pre = ldv(in + in_index(C.c - a.M + C.mm));
+    C.template levels_from<1>();
+    // advance the rings by one chunk (descending march: head -= M mod Q)
 #pragma unroll
-        for (int it = 0; it < ITEMS; ++it) {
-          int q = tid + it * NT;
-          if (q >= MR) break;
-          int mm = q >> logR, r = q & (R - 1);
-          int64_t m = pmod(cn + mm, Nm);
-          pre[it] = in[r0 + r + (m << j0)];
-        }
-      }
-
-#pragma unroll
-      for (int it = 0; it < ITEMS; ++it) {
-        int q = tid + it * NT;
-        if (q >= MR) break;
-        int mm = q >> logR, r = q & (R - 1);
-        int slot = headin + mm;
-        if (slot >= Qin) slot -= Qin;
-        T sv, dv;
-        if constexpr (FILT == kHAAR) {
-          T v0 = lds[offin + slot * R + r];
-          int s1 = slot + s;
-          if (s1 >= Qin) s1 -= Qin;
-          T v1 = lds[offin + s1 * R + r];
-          sv = (v0 + v1) * T(kInvR2);
-          dv = (v0 - v1) * T(kInvR2);
-        } else {
-          T v = lds[offin + slot * R + r];
-          sv = v * T(Filter<FILT>::c(0));
-          dv = v * T(gcoef<FILT>(0));
-#pragma unroll
-          for (int kk = 1; kk < L; ++kk) {
-            slot += s;
-            if (slot >= Qin) slot -= Qin;
-            v = lds[offin + slot * R + r];
-            sv = fma(v, T(Filter<FILT>::c(kk)), sv);
-            dv = fma(v, T(gcoef<FILT>(kk)), dv);
-          }
-        }
-        if (store) {
-          int so = headout + mm;
-          if (so >= Qout) so -= Qout;
-          lds[offout + so * R + r] = sv;
-        }
-        const int64_t m = c + mm;
-        if (m >= mlo && m < mhi) {
-          const int64_t i = r0 + r + (m << j0);
-          __builtin_nontemporal_store(sv, oS + i);
-          __builtin_nontemporal_store(dv, oD + i);
-        }
+    for (int l = 0; l < kGmax; ++l) {
+      const int Q = a.M + fwd_win(L, l);
+      int h = C.head[l] - a.M;
+      while (h < 0) h += Q;
+      C.head[l] = h;
+    }
+    if (have_next) {
+      if (C.active) {
+        const int Q0 = a.M + fwd_win(L, 0);
+        int so = C.head[0] + C.mm;
+        if (so >= Q0) so -= Q0;
+        stv(C.lds + size_t(so) * R + C.r, pre);
       }
       __syncthreads();
-      if (l == 1 && have_next) {
-        // ring 0 slots of chunk k+1: head0(k+1) + mm
-        const int Q0 = M + (L - 1);
-        const int h0 = (Q0 - int((int64_t(k + 1) * M) % Q0)) % Q0;
-#pragma unroll
-        for (int it = 0; it < ITEMS; ++it) {
-          int q = tid + it * NT;
-          if (q >= MR) break;
-          int mm = q >> logR, r = q & (R - 1);
-          int so = h0 + mm;
-          if (so >= Q0) so -= Q0;
-          lds[so * R + r] = pre[it];
-        }
-        if (g == 1) __syncthreads();
-      }
     }
   }
 }
 
 // ---------------------------------------------------------------- inverse
-template <typename T, int FILT, int NT, int ITEMS>
-__global__ __launch_bounds__(NT) void modwt_inv_group(ModwtArgs a) {
-  constexpr int L = Filter<FILT>::L;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* lds = reinterpret_cast<T*>(smem_raw);
+template <typename T, int FILT, int R, int NT>
+struct InvCtx {
+  using V = typename Vec2<T>::type;
+  static constexpr int L = Filter<FILT>::L;
+  T* lds;
+  int baseD;
+  const ModwtArgs* a;
+  const T* xmod;
+  const T* stop;
+  T* out;
+  int M, mm, r, g, j0;
+  bool active;
+  int64_t n, Nm, r0, mlo, mhi, c;
+  int head[kGmax + 1];  // head of ring l (1..g): slot of position c - W'_l
+  V pre[kGmax + 1];     // next chunk: D_l (l = 1..g) and S_g (index 0)
 
-  const int tid = threadIdx.x;
-  const int logR = a.logR, R = 1 << logR, M = a.M, MR = M << logR;
-  const int64_t n = a.n, Nm = n >> a.j0;
-  const int64_t nrb = (int64_t(1) << a.j0) >> logR;
-  const int64_t rb = int64_t(blockIdx.x) % nrb, seg = int64_t(blockIdx.x) / nrb;
-  const int64_t r0 = rb << logR;
-  const int64_t mlo = seg * a.P, mhi = mlo + a.P;
-  const int g = a.g, j0 = a.j0;
-  const T* __restrict__ xmod = static_cast<const T*>(a.xmod);
-  const T* __restrict__ stop = static_cast<const T*>(a.stop);
-  T* __restrict__ out = static_cast<T*>(a.out);
-  // ring l (1..g) size Q_l = M + W_l, W_l = (L-1) 2^{l-1}; S rings first,
-  // then D rings.
-  const int baseD = (g * M + (L - 1) * ((1 << g) - 1)) * R;
-  const int64_t cbot = mlo - a.warm;  // first chunk start (ascending march)
-
-  auto thr_of = [&](int l) -> T {  // threshold of group level l (1-based)
+  __device__ __forceinline__ int64_t gidx(int64_t m) const {
+    m &= (Nm - 1);
+    return (R == 1) ? m : r0 + r + (m << j0);
+  }
+  __device__ __forceinline__ T thr(int l) const {
     const int lev = j0 + l - 1;
-    if (!a.th_on || lev < a.minlevel || lev > a.maxlevel) return T(-1);
-    return a.th_levels ? T(a.th_levels[lev]) : T(a.th_value);
-  };
+    if (!a->th_on || lev < a->minlevel || lev > a->maxlevel) return T(-1);
+    return a->th_levels ? T(a->th_levels[lev]) : T(a->th_value);
+  }
+  __device__ __forceinline__ V thr2(V d, T t) const {
+    if (t >= T(0)) {
+      d.x = thresh_coef(d.x, t, a->hardness);
+      d.y = thresh_coef(d.y, t, a->hardness);
+    }
+    return d;
+  }
 
-  // prologue: every ring's chunk-0 region (offsets W_l + mm, head 0)
-  for (int l = 1; l <= g; ++l) {
-    const int W = (L - 1) << (l - 1);
-    const int offS = ((l - 1) * M + (L - 1) * ((1 << (l - 1)) - 1)) * R;
-    const T* __restrict__ dsrc = xmod + 2 * n * int64_t(j0 + l - 1) + n;
-    const T t = thr_of(l);
-#pragma unroll
-    for (int it = 0; it < ITEMS; ++it) {
-      int q = tid + it * NT;
-      if (q >= MR) break;
-      int mm = q >> logR, r = q & (R - 1);
-      int64_t m = pmod(cbot + mm, Nm);
-      int64_t i = r0 + r + (m << j0);
-      T d = dsrc[i];
-      if (t >= T(0)) d = thresh_coef(d, t, a.hardness);
-      lds[baseD + offS + (W + mm) * R + r] = d;
-      if (l == g) lds[offS + (W + mm) * R + r] = stop[i];
+  // issue the HBM loads of chunk cc for ring l (l = 1..g)
+  template <int l>
+  __device__ __forceinline__ void load_next(int64_t cc) {
+    if constexpr (l <= kGmax) {
+      if (l <= g) {
+        if (active) {
+          const int64_t i = gidx(cc + mm);
+          pre[l] = ldv(xmod + 2 * n * int64_t(j0 + l - 1) + n + i);
+          if (l == g) pre[0] = ldv(stop + i);
+        }
+        load_next<l + 1>(cc);
+      }
     }
   }
-  __syncthreads();
-
-  for (int k = 0; k < a.nchunks; ++k) {
-    const int64_t c = cbot + int64_t(k) * M;
-    const bool have_next = (k + 1 < a.nchunks);
-    for (int l = g; l >= 1; --l) {
-      const int s = 1 << (l - 1);
-      const int W = (L - 1) * s;
-      const int Q = M + W;
-      const int offS = ((l - 1) * M + (L - 1) * (s - 1)) * R;
-      const int head = int((int64_t(k) * M) % Q);
-      T dpre[ITEMS], spre[ITEMS];
-      const T t = thr_of(l);
-      const T* __restrict__ dsrc = xmod + 2 * n * int64_t(j0 + l - 1) + n;
-      if (have_next) {
-        const int64_t cn = c + M;
-#pragma unroll
-        for (int it = 0; it < ITEMS; ++it) {
-          int q = tid + it * NT;
-          if (q >= MR) break;
-          int mm = q >> logR, r = q & (R - 1);
-          int64_t m = pmod(cn + mm, Nm);
-          int64_t i = r0 + r + (m << j0);
-          dpre[it] = dsrc[i];
-          if (l == g) spre[it] = stop[i];
+  // park the loaded chunk in the rings (slots head_l + W'_l + mm)
+  template <int l>
+  __device__ __forceinline__ void park() {
+    if constexpr (l <= kGmax) {
+      if (l <= g) {
+        if (active) {
+          const int Q = M + inv_win(L, l);
+          int so = head[l] + inv_win(L, l) + mm;
+          if (so >= Q) so -= Q;
+          const size_t off = size_t(inv_off(L, M, l)) * R + size_t(so) * R + r;
+          stv(lds + baseD + off, thr2(pre[l], thr(l)));
+          if (l == g) stv(lds + off, pre[0]);
         }
+        park<l + 1>();
       }
-      // output ring (level l-1): Q_{l-1} = M + W/2, head_{l-1}
-      const int Wo = (l > 1) ? (L - 1) * (s >> 1) : 0;
-      const int Qo = M + Wo;
-      const int offSo = (l > 1) ? ((l - 2) * M + (L - 1) * ((s >> 1) - 1)) * R : 0;
-      const int heado = (l > 1) ? int((int64_t(k) * M) % Qo) : 0;
+    }
+  }
+
+  // level l: S_l, D_l (ring l) -> S_{l-1} (ring l-1, or HBM when l == 1)
+  template <int l>
+  __device__ __forceinline__ void level() {
+    constexpr int s = 1 << (l - 1);
+    const int W = inv_win(L, l);
+    const int Q = M + W;
+    const T* rs = lds + size_t(inv_off(L, M, l)) * R;
+    const T* rd = rs + baseD;
+    if (active) {
+      T e0, e1;
+      // tap k of element m sits at slot head + W + mm - k s
+      if constexpr (R == 1 && s == 1) {
+        // pairs from slot head + mm (even): w[i] = slot head + mm + i
+        T ws[L + 2], wd[L + 2];
+        int sl = head[l] + mm;
+        if (sl >= Q) sl -= Q;
 #pragma unroll
-      for (int it = 0; it < ITEMS; ++it) {
-        int q = tid + it * NT;
-        if (q >= MR) break;
-        int mm = q >> logR, r = q & (R - 1);
-        // position m = c + mm sits at offset W + mm; tap k at offset W + mm - k s
-        T acc;
+        for (int q = 0; q <= L / 2; ++q) {
+          int sq = sl + 2 * q;
+          if (sq >= Q) sq -= Q;
+          V vs = ldv(rs + sq), vd = ldv(rd + sq);
+          ws[2 * q] = vs.x;
+          ws[2 * q + 1] = vs.y;
+          wd[2 * q] = vd.x;
+          wd[2 * q + 1] = vd.y;
+        }
+        // element m: tap k at w[L - k]; element m+1 at w[L + 1 - k]
+        // (the level-1 window is L - 1 rounded up to L)
         if constexpr (FILT == kHAAR) {
-          int s0 = head + W + mm;
-          if (s0 >= Q) s0 -= Q;
-          int s1 = s0 - s;
-          if (s1 < 0) s1 += Q;
-          T S0 = lds[offS + s0 * R + r], D0 = lds[baseD + offS + s0 * R + r];
-          T S1 = lds[offS + s1 * R + r], D1 = lds[baseD + offS + s1 * R + r];
-          acc = (S0 + D0 + S1 - D1) * T(kHInvR2);
+          e0 = (ws[L] + wd[L] + ws[L - 1] - wd[L - 1]) * T(kHInvR2);
+          e1 = (ws[L + 1] + wd[L + 1] + ws[L] - wd[L]) * T(kHInvR2);
         } else {
-          // term order k = L-2, L-4, ..., 0, L-1, ..., 1 (src/daub4.cpp:578-583)
-          acc = T(0);
           bool first = true;
+          e0 = T(0);
+          e1 = T(0);
 #pragma unroll
           for (int pass = 0; pass < 2; ++pass) {
 #pragma unroll
             for (int kk = L - 2 + pass; kk >= 0; kk -= 2) {
-              int so = head + W + mm - kk * s;
-              if (so >= Q) so -= Q;
-              if (so < 0) so += Q;
-              T Sv = lds[offS + so * R + r], Dv = lds[baseD + offS + so * R + r];
+              const T cc = T(Filter<FILT>::c(kk)), gk = T(gcoef<FILT>(kk));
               if (first) {
-                acc = Sv * T(Filter<FILT>::c(kk));
+                e0 = ws[L - kk] * cc;
+                e1 = ws[L + 1 - kk] * cc;
                 first = false;
               } else {
-                acc = fma(Sv, T(Filter<FILT>::c(kk)), acc);
+                e0 = fma(ws[L - kk], cc, e0);
+                e1 = fma(ws[L + 1 - kk], cc, e1);
               }
-              acc = fma(Dv, T(gcoef<FILT>(kk)), acc);
+              e0 = fma(wd[L - kk], gk, e0);
+              e1 = fma(wd[L + 1 - kk], gk, e1);
             }
           }
-          acc = T(0.5) * acc;
+          e0 = T(0.5) * e0;
+          e1 = T(0.5) * e1;
         }
-        if (l > 1) {
-          int so = heado + Wo + mm;
-          if (so >= Qo) so -= Qo;
-          lds[offSo + so * R + r] = acc;
+      } else {
+        int sl = head[l] + W + mm;
+        if (sl >= Q) sl -= Q;
+        if constexpr (FILT == kHAAR) {
+          int s1 = sl - s;
+          if (s1 < 0) s1 += Q;
+          V S0 = ldv(rs + size_t(sl) * R + r), D0 = ldv(rd + size_t(sl) * R + r);
+          V S1 = ldv(rs + size_t(s1) * R + r), D1 = ldv(rd + size_t(s1) * R + r);
+          e0 = (S0.x + D0.x + S1.x - D1.x) * T(kHInvR2);
+          e1 = (S0.y + D0.y + S1.y - D1.y) * T(kHInvR2);
         } else {
-          const int64_t m = c + mm;
-          if (m >= mlo && m < mhi) __builtin_nontemporal_store(acc, out + r0 + r + (m << j0));
-        }
-      }
-      __syncthreads();
-      if (have_next) {
-        // chunk k+1 region of ring l: head(k+1) + W + mm
-        const int hn = int((int64_t(k + 1) * M) % Q);
+          V vs[L], vd[L];
 #pragma unroll
-        for (int it = 0; it < ITEMS; ++it) {
-          int q = tid + it * NT;
-          if (q >= MR) break;
-          int mm = q >> logR, r = q & (R - 1);
-          int so = hn + W + mm;
-          if (so >= Q) so -= Q;
-          T d = dpre[it];
-          if (t >= T(0)) d = thresh_coef(d, t, a.hardness);
-          lds[baseD + offS + so * R + r] = d;
-          if (l == g) lds[offS + so * R + r] = spre[it];
+          for (int kk = 0; kk < L; ++kk) {
+            int sk = sl - kk * s;
+            if (sk < 0) sk += Q;
+            vs[kk] = ldv(rs + size_t(sk) * R + r);
+            vd[kk] = ldv(rd + size_t(sk) * R + r);
+          }
+          bool first = true;
+          e0 = T(0);
+          e1 = T(0);
+          // term order k = L-2, L-4, ..., 0, L-1, ..., 1 (src/daub4.cpp:578-583)
+#pragma unroll
+          for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+            for (int kk = L - 2 + pass; kk >= 0; kk -= 2) {
+              const T cc = T(Filter<FILT>::c(kk)), gk = T(gcoef<FILT>(kk));
+              if (first) {
+                e0 = vs[kk].x * cc;
+                e1 = vs[kk].y * cc;
+                first = false;
+              } else {
+                e0 = fma(vs[kk].x, cc, e0);
+                e1 = fma(vs[kk].y, cc, e1);
+              }
+              e0 = fma(vd[kk].x, gk, e0);
+              e1 = fma(vd[kk].y, gk, e1);
+            }
+          }
+          e0 = T(0.5) * e0;
+          e1 = T(0.5) * e1;
         }
-        if (g == 1) __syncthreads();
       }
+      if (l > 1) {
+        const int Wo = inv_win(L, l - 1);
+        const int Qo = M + Wo;
+        int so = head[l - 1] + Wo + mm;
+        if (so >= Qo) so -= Qo;
+        stv(lds + size_t(inv_off(L, M, l - 1)) * R + size_t(so) * R + r, V{e0, e1});
+      } else {
+        const int64_t m = c + mm;
+        if (m >= mlo && m < mhi) {
+          const int64_t i = (R == 1) ? m : r0 + r + (m << j0);
+          stv_nt(out + i, V{e0, e1});
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  template <int l>
+  __device__ __forceinline__ void levels_down() {
+    if constexpr (l >= 1) {
+      if (l <= g) level<l>();
+      levels_down<l - 1>();
+    }
+  }
+};
+
+template <typename T, int FILT, int R, int NT>
+__global__ __launch_bounds__(NT) void modwt_inv_v2(ModwtArgs a) {
+  constexpr int L = Filter<FILT>::L;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  InvCtx<T, FILT, R, NT> C;
+  C.lds = reinterpret_cast<T*>(smem_raw);
+  C.a = &a;
+  C.xmod = static_cast<const T*>(a.xmod);
+  C.stop = static_cast<const T*>(a.stop);
+  C.out = static_cast<T*>(a.out);
+  C.M = a.M;
+  C.g = a.g;
+  C.j0 = a.j0;
+  C.n = a.n;
+  C.Nm = a.n >> a.j0;
+  C.baseD = inv_off(L, a.M, a.g + 1) * R;
+  const int e = 2 * int(threadIdx.x);
+  C.active = e < a.M * R;
+  C.mm = e / R;
+  C.r = e % R;
+  const int64_t nrb = (int64_t(1) << a.j0) / R;
+  const int64_t rb = int64_t(blockIdx.x) % nrb, seg = int64_t(blockIdx.x) / nrb;
+  C.r0 = rb * R;
+  C.mlo = seg * a.P;
+  C.mhi = C.mlo + a.P;
+  const int64_t cbot = C.mlo - a.warm;
+#pragma unroll
+  for (int l = 0; l <= kGmax; ++l) C.head[l] = 0;
+
+  // prologue: chunk 0 of every ring
+  C.template load_next<1>(cbot);
+  C.template park<1>();
+  __syncthreads();
+
+  for (int k = 0; k < a.nchunks; ++k) {
+    C.c = cbot + int64_t(k) * a.M;
+    const bool have_next = (k + 1 < a.nchunks);
+    if (have_next) C.template load_next<1>(C.c + a.M);
+    C.template levels_down<kGmax>();
+    // advance rings (ascending march: head += M mod Q)
+#pragma unroll
+    for (int l = 1; l <= kGmax; ++l) {
+      const int Q = a.M + inv_win(L, l);
+      int h = C.head[l] + a.M;
+      while (h >= Q) h -= Q;
+      C.head[l] = h;
+    }
+    if (have_next) {
+      C.template park<1>();
+      __syncthreads();
     }
   }
 }
@@ -322,38 +516,29 @@ __global__ __launch_bounds__(NT) void modwt_inv_group(ModwtArgs a) {
 
 namespace {
 
-constexpr int kNT = 256;
-constexpr int kRmax = 32;
-
-// positions per thread per chunk: the inverse keeps two rings per level
-// (S and D), so it runs half-size chunks to fit 10 fused LA8 levels in LDS
-template <typename T, bool INV>
-constexpr int items_for() {
-  return (sizeof(T) == 8 ? 2 : 4) / (INV ? 2 : 1);
-}
+constexpr int kNT1 = 512;   // threads, group starting at level 1 (R = 1)
+constexpr int kNTR = 256;   // threads, deeper groups (R = 32)
+constexpr int kRdeep = 32;
 
 struct Group {
-  int j0, g, logR, M;
+  int j0, g, R, NT, M;
   int64_t P;
   int nseg, warm, nchunks;
   size_t lds;
 };
 
-size_t lds_bytes(bool inverse, int L, int g, int M, int R, size_t esz) {
-  size_t per = size_t(g) * M + size_t(L - 1) * ((size_t(1) << g) - 1);
-  return (inverse ? 2 : 1) * per * R * esz;
+size_t ring_elems(bool inverse, int L, int g, int M) {
+  return inverse ? size_t(2) * inv_off(L, M, g + 1) : size_t(fwd_off(L, M, g));
 }
 
 int64_t round_up64(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
-// Partition levels 1..J into groups; per group choose residues per tile,
-// chunk length, fused depth and segment count.  Depth is limited by the
-// LDS budget and by keeping the warm-up at most half a segment when the
-// grid fills every CU.
-std::vector<Group> plan_groups(bool inverse, int64_t n, int J, int L, size_t esz, int items) {
+// Partition levels 1..J into groups.  The first group (R = 1) fuses as many
+// levels as the LDS budget allows while its warm-up stays at most half a
+// segment with one workgroup per CU; deeper groups run R = 32 residues.
+std::vector<Group> plan_groups(bool inverse, int64_t n, int J, int L, size_t esz) {
   const DeviceInfo& dev = device_info();
-  const size_t budget = std::min<size_t>(dev.max_lds_per_block, 160 * 1024);
-  const int64_t CH = int64_t(kNT) * items;  // positions per chunk
+  const size_t budget = std::min<size_t>(dev.max_lds_per_block, 160 * 1024) - 1024;
   const int64_t cus = std::max(1, dev.num_cus);
   std::vector<Group> gs;
   int j0 = 0;
@@ -361,19 +546,23 @@ std::vector<Group> plan_groups(bool inverse, int64_t n, int J, int L, size_t esz
     Group G{};
     G.j0 = j0;
     const int64_t B = int64_t(1) << j0;
-    const int R = int(std::min<int64_t>(B, kRmax));
-    G.logR = 0;
-    while ((1 << G.logR) < R) ++G.logR;
+    // R = 1 for the first group; deeper groups take 32 residues when the
+    // stride allows it (B >= 32), else all B of them
+    G.R = (j0 == 0) ? 1 : int(std::min<int64_t>(B, kRdeep));
+    G.NT = (G.R == 1) ? kNT1 : kNTR;
     const int64_t Nm = n >> j0;
-    G.M = int(std::min<int64_t>(CH / R, Nm));
-    const int64_t nrb = B / R;
+    G.M = int(std::min<int64_t>(2 * G.NT / G.R, Nm));
+    const int64_t nrb = B / G.R;
     const int64_t target = std::max<int64_t>(1, cus / nrb);
     int64_t nseg_t = 1;
     while (nseg_t * 2 <= target && Nm / (nseg_t * 2) >= G.M) nseg_t *= 2;
     const int64_t Pt = Nm / nseg_t;
-    int gbest = 1;
-    for (int g = 2; g <= J - j0; ++g) {
-      if (lds_bytes(inverse, L, g, G.M, R, esz) > budget) break;
+    // the first group always fuses at least min(J, 5) levels so that deeper
+    // groups start at a stride of 32 or more
+    const int gmin = (j0 == 0) ? std::min(J, 5) : 1;
+    int gbest = gmin;
+    for (int g = gmin + 1; g <= std::min(J - j0, kGmax); ++g) {
+      if (ring_elems(inverse, L, g, G.M) * G.R * esz > budget) break;
       int64_t warm = round_up64(int64_t(L - 1) * ((int64_t(1) << g) - 1), G.M);
       if (warm > Pt / 2 && nseg_t > 1) break;
       gbest = g;
@@ -387,7 +576,7 @@ std::vector<Group> plan_groups(bool inverse, int64_t n, int J, int L, size_t esz
     G.nseg = int(nseg);
     G.P = Nm / nseg;
     G.nchunks = int((G.P + G.warm) / G.M);
-    G.lds = lds_bytes(inverse, L, G.g, G.M, R, esz);
+    G.lds = ring_elems(inverse, L, G.g, G.M) * G.R * esz;
     gs.push_back(G);
     j0 += G.g;
   }
@@ -400,32 +589,39 @@ void allow_big_lds(KERN kern) {
                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 }
 
-template <typename T, int FILT>
-int launch_fwd(const Group& G, const ModwtArgs& a, hipStream_t st) {
-  constexpr int IT = items_for<T, false>();
-  auto kern = modwt_fwd_group<T, FILT, kNT, IT>;
+template <typename T, int FILT, int R, int NT, bool INV>
+int launch_group(const Group& G, const ModwtArgs& a, hipStream_t st) {
+  auto kern = INV ? modwt_inv_v2<T, FILT, R, NT> : modwt_fwd_v2<T, FILT, R, NT>;
   static std::once_flag once;
   std::call_once(once, [&] { allow_big_lds(kern); });
-  int64_t nrb = (int64_t(1) << G.j0) >> G.logR;
+  int64_t nrb = (int64_t(1) << G.j0) / G.R;
   int64_t blocks = nrb * G.nseg;
   if (blocks > 0x7fffffff) return set_error(WH_ERR_ARG, "modwt: grid too large");
-  hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(kNT), G.lds, st, a);
+  hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(NT), G.lds, st, a);
   WH_HIP_CHECK(hipGetLastError());
   return WH_OK;
 }
 
-template <typename T, int FILT>
-int launch_inv(const Group& G, const ModwtArgs& a, hipStream_t st) {
-  constexpr int IT = items_for<T, true>();
-  auto kern = modwt_inv_group<T, FILT, kNT, IT>;
-  static std::once_flag once;
-  std::call_once(once, [&] { allow_big_lds(kern); });
-  int64_t nrb = (int64_t(1) << G.j0) >> G.logR;
-  int64_t blocks = nrb * G.nseg;
-  if (blocks > 0x7fffffff) return set_error(WH_ERR_ARG, "modwt: grid too large");
-  hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(kNT), G.lds, st, a);
-  WH_HIP_CHECK(hipGetLastError());
-  return WH_OK;
+template <typename T, int FILT, bool INV>
+int launch_r(const Group& G, const ModwtArgs& a, hipStream_t st) {
+  switch (G.R) {
+    case 1: return launch_group<T, FILT, 1, kNT1, INV>(G, a, st);
+    case 2: return launch_group<T, FILT, 2, kNTR, INV>(G, a, st);
+    case 4: return launch_group<T, FILT, 4, kNTR, INV>(G, a, st);
+    case 8: return launch_group<T, FILT, 8, kNTR, INV>(G, a, st);
+    case 16: return launch_group<T, FILT, 16, kNTR, INV>(G, a, st);
+    default: return launch_group<T, FILT, kRdeep, kNTR, INV>(G, a, st);
+  }
+}
+
+template <typename T, bool INV>
+int launch_any(int filt, const Group& G, const ModwtArgs& a, hipStream_t st) {
+  switch (filt) {
+    case kHAAR: return launch_r<T, kHAAR, INV>(G, a, st);
+    case kD4: return launch_r<T, kD4, INV>(G, a, st);
+    case kC6: return launch_r<T, kC6, INV>(G, a, st);
+    default: return launch_r<T, kLA8, INV>(G, a, st);
+  }
 }
 
 ModwtArgs args_of(const Group& G, int64_t n) {
@@ -434,7 +630,6 @@ ModwtArgs args_of(const Group& G, int64_t n) {
   a.P = G.P;
   a.j0 = G.j0;
   a.g = G.g;
-  a.logR = G.logR;
   a.M = G.M;
   a.warm = G.warm;
   a.nchunks = G.nchunks;
@@ -444,18 +639,12 @@ ModwtArgs args_of(const Group& G, int64_t n) {
 template <typename T>
 int modwt_fwd_t(const void* x, void* xmod, int64_t n, int J, int filt, hipStream_t st) {
   const int L = filter_length(filt);
-  auto groups = plan_groups(false, n, J, L, sizeof(T), items_for<T, false>());
+  auto groups = plan_groups(false, n, J, L, sizeof(T));
   for (const Group& G : groups) {
     ModwtArgs a = args_of(G, n);
     a.in = (G.j0 == 0) ? x : static_cast<const T*>(xmod) + 2 * n * int64_t(G.j0 - 1);
     a.xmod = xmod;
-    int rc;
-    switch (filt) {
-      case kHAAR: rc = launch_fwd<T, kHAAR>(G, a, st); break;
-      case kD4: rc = launch_fwd<T, kD4>(G, a, st); break;
-      case kC6: rc = launch_fwd<T, kC6>(G, a, st); break;
-      default: rc = launch_fwd<T, kLA8>(G, a, st); break;
-    }
+    int rc = launch_any<T, false>(filt, G, a, st);
     if (rc) return rc;
   }
   return WH_OK;
@@ -465,7 +654,7 @@ template <typename T>
 int modwt_inv_t(const void* xmod, void* x, int64_t n, int J, int filt, const wh_thresh* thr,
                 hipStream_t st) {
   const int L = filter_length(filt);
-  auto groups = plan_groups(true, n, J, L, sizeof(T), items_for<T, true>());
+  auto groups = plan_groups(true, n, J, L, sizeof(T));
   // the S_{j0} handed between groups lives in library scratch (2 n values)
   T* scratch = nullptr;
   if (groups.size() > 1) {
@@ -490,13 +679,7 @@ int modwt_inv_t(const void* xmod, void* x, int64_t n, int J, int filt, const wh_
       a.th_value = thr->value;
       a.th_levels = thr->levels;
     }
-    int rc;
-    switch (filt) {
-      case kHAAR: rc = launch_inv<T, kHAAR>(G, a, st); break;
-      case kD4: rc = launch_inv<T, kD4>(G, a, st); break;
-      case kC6: rc = launch_inv<T, kC6>(G, a, st); break;
-      default: rc = launch_inv<T, kLA8>(G, a, st); break;
-    }
+    int rc = launch_any<T, true>(filt, G, a, st);
     if (rc) return rc;
     stop = dst;
   }
@@ -519,17 +702,14 @@ int modwt_inverse(const void* xmod, void* x, int64_t n, int J, int filt, int dty
 
 std::string modwt_plan_string(bool inverse, int64_t n, int J, int filt, int dtype) {
   const int L = filter_length(filt);
-  const bool f32 = dtype == WH_F32;
-  size_t esz = f32 ? 4 : 8;
-  int items = f32 ? (inverse ? items_for<float, true>() : items_for<float, false>())
-                  : (inverse ? items_for<double, true>() : items_for<double, false>());
-  auto gs = plan_groups(inverse, n, J, L, esz, items);
+  size_t esz = dtype == WH_F32 ? 4 : 8;
+  auto gs = plan_groups(inverse, n, J, L, esz);
   std::string s;
   for (auto& G : gs) {
     char buf[256];
     snprintf(buf, sizeof buf,
-             "{j0=%d g=%d R=%d M=%d nseg=%d P=%lld warm=%d chunks=%d lds=%zu} ", G.j0, G.g,
-             1 << G.logR, G.M, G.nseg, (long long)G.P, G.warm, G.nchunks, G.lds);
+             "{j0=%d g=%d R=%d NT=%d M=%d nseg=%d P=%lld warm=%d chunks=%d lds=%zu} ", G.j0,
+             G.g, G.R, G.NT, G.M, G.nseg, (long long)G.P, G.warm, G.nchunks, G.lds);
     s += buf;
   }
   return s;
