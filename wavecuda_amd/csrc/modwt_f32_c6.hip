@@ -1,0 +1,10 @@
+// Instantiation unit of the fused MODWT kernels: element type float, filter C6.
+#include "modwt_launch.h"
+
+namespace wh {
+template <>
+int modwt_launch<float, kC6>(bool inverse, const GroupLaunch& gl, const ModwtArgs& a,
+                            hipStream_t st) {
+  return modwt_launch_impl<float, kC6>(inverse, gl, a, st);
+}
+}  // namespace wh

@@ -1,0 +1,61 @@
+// modwt_launch.h -- kernel selection for one (element type, filter); included
+// by the modwt_<type>_<filter>.hip instantiation units only.
+#pragma once
+
+#include <mutex>
+
+#include "modwt_kern.h"
+#include "wh_internal.h"
+
+namespace wh {
+
+template <typename KERN>
+inline void modwt_allow_lds(KERN kern) {
+  static std::once_flag once;
+  std::call_once(once, [&] {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  });
+}
+
+template <typename T, int FILT, int R, int NT, int G, int MAP>
+int modwt_launch_map(bool inverse, const GroupLaunch& gl, const ModwtArgs& a, hipStream_t st) {
+  if (inverse) {
+    auto kern = modwt_inv_k<T, FILT, R, NT, G, MAP>;
+    modwt_allow_lds(kern);
+    hipLaunchKernelGGL(kern, dim3(unsigned(gl.blocks)), dim3(NT), gl.lds, st, a);
+  } else {
+    auto kern = modwt_fwd_k<T, FILT, R, NT, G, MAP>;
+    modwt_allow_lds(kern);
+    hipLaunchKernelGGL(kern, dim3(unsigned(gl.blocks)), dim3(NT), gl.lds, st, a);
+  }
+  WH_HIP_CHECK(hipGetLastError());
+  return WH_OK;
+}
+
+template <typename T, int FILT, int R, int NT, int G>
+int modwt_launch_one(bool inverse, const GroupLaunch& gl, const ModwtArgs& a, hipStream_t st) {
+  if constexpr (R == 1) {
+    if (gl.map == 1) return modwt_launch_map<T, FILT, R, NT, G, 1>(inverse, gl, a, st);
+  }
+  return modwt_launch_map<T, FILT, R, NT, G, 0>(inverse, gl, a, st);
+}
+
+template <typename T, int FILT, int R, int NT, int G>
+int modwt_launch_g(bool inverse, const GroupLaunch& gl, const ModwtArgs& a, hipStream_t st) {
+  if constexpr (G > kGmax) {
+    return set_error(WH_ERR_UNSUPPORTED, "modwt: group depth out of range");
+  } else {
+    if (gl.G == G) return modwt_launch_one<T, FILT, R, NT, G>(inverse, gl, a, st);
+    return modwt_launch_g<T, FILT, R, NT, G + 1>(inverse, gl, a, st);
+  }
+}
+
+template <typename T, int FILT>
+int modwt_launch_impl(bool inverse, const GroupLaunch& gl, const ModwtArgs& a, hipStream_t st) {
+  if (gl.R == 1 && gl.NT == 512) return modwt_launch_g<T, FILT, 1, 512, 1>(inverse, gl, a, st);
+  if (gl.R == 32 && gl.NT == 256) return modwt_launch_g<T, FILT, 32, 256, 1>(inverse, gl, a, st);
+  return set_error(WH_ERR_UNSUPPORTED, "modwt: unsupported residue tiling");
+}
+
+}  // namespace wh
