@@ -43,7 +43,7 @@ def traffic_from_profiles():
     """HBM bytes per launch of the forward kernel measured by rocprofv3
     FETCH_SIZE / WRITE_SIZE (scaled by a copy-kernel calibration), if the
     committed summary exists."""
-    p = ROOT / "profiles" / "modwt_fwd_la8_n2p24_J10_traffic.txt"
+    p = ROOT / "profiles" / "modwt_la8_n2p24_J10_traffic.txt"
     if not p.exists():
         return None
     m = re.search(r"^hbm_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text(), re.M)
@@ -78,6 +78,96 @@ def cpu_baseline(n_sample=1 << 22, J=10, filt=4):
     }
 
 
+def batch_bytes(lens, ttypes, levels, filters, esz):
+    """Words a fused smoothing pipeline must move per signal, summed: the
+    DWT reads x, writes and re-reads its n coefficients, writes y (4 n);
+    the MODWT reads x, writes D_1..D_J and S_J, re-reads them, writes y
+    ((2J + 4) n)."""
+    from wavecuda_amd import _lib as W
+
+    tot = 0
+    for n, tt, lv, f in zip(lens, ttypes, levels, filters):
+        n = int(n)
+        if tt == 0:
+            tot += 4 * n
+        else:
+            J = W.lib().wh_resolve_levels(n, int(lv), int(f))
+            tot += (2 * J + 4) * n
+    return tot * esz
+
+
+def batch_section(steps, warmup, plain_sample):
+    """Batched smoothing of the mixed 8192-signal workload (one persistent
+    launch, replayed from a HIP graph) against the plain per-signal GPU
+    loop (one universal-threshold smoothing call per signal)."""
+    import numpy as np
+    import torch
+
+    from wavecuda_amd import _lib as W
+    from wavecuda_amd import device as D
+    from wavecuda_amd.batch import BatchPlan
+    from wavecuda_amd.signals import batch_workload
+
+    x, lens, filters, ttypes, levels = batch_workload()
+    out = {"workload": "8192 doppler+noise signals, n=2^k k~U{8..16}, filter~U{Haar,D4,C6,LA8}, "
+                       "DWT full depth or MODWT J=min(8,k-2), SNR~U[1,10]; "
+                       "forward -> universal soft threshold -> inverse",
+           "signals": int(len(lens)), "samples": int(lens.astype(np.int64).sum())}
+    for dtype, tag in ((torch.float64, "f64"), (torch.float32, "f32")):
+        esz = 8 if dtype == torch.float64 else 4
+        plan = BatchPlan(lens, filters, ttypes, levels, dtype=dtype)
+        xg = torch.tensor(x, dtype=dtype, device="cuda")
+        y, t = plan.tune(xg)
+        plan.capture(xg, y, t)
+        for _ in range(warmup):
+            plan.replay()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(steps)]
+        for a, b in ev:
+            a.record()
+            plan.replay()
+            b.record()
+        torch.cuda.synchronize()
+        ms = min(a.elapsed_time(b) for a, b in ev)
+        ms_avg = sum(a.elapsed_time(b) for a, b in ev) / steps
+        # plain loop over a leading sample of the signals (in input order),
+        # outputs compared bit for bit with the batched ones
+        o = plan.offsets
+        m = min(plain_sample, len(lens))
+        ys = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(m):
+            yi, ti = D.smooth(xg[o[i]:o[i + 1]], int(levels[i]), int(ttypes[i]),
+                              int(filters[i]), W.UNIV)
+            ys.append((yi, ti))
+        torch.cuda.synchronize()
+        plain_s = time.perf_counter() - t0
+        same = all(torch.equal(yi, y[o[i]:o[i + 1]]) and ti == t[i].item()
+                   for i, (yi, ti) in enumerate(ys))
+        frac_samples = float(o[m]) / float(o[-1])
+        plain_full_ms = plain_s * 1e3 / frac_samples if m < len(lens) else plain_s * 1e3
+        nbytes = batch_bytes(lens, ttypes, levels, filters, esz)
+        info = plan.info()
+        out[tag] = {
+            "batched_ms": round(ms_avg, 3),
+            "batched_ms_min": round(ms, 3),
+            "msamples_per_s": round(float(o[-1]) / (ms_avg * 1e-3) / 1e6, 1),
+            "hbm_frac": round(nbytes / (ms_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "pipeline_bytes": nbytes,
+            "plain_loop_signals": m,
+            "plain_loop_ms": round(plain_s * 1e3, 1),
+            "plain_loop_ms_full_est": round(plain_full_ms, 1),
+            "speedup_vs_plain": round(plain_full_ms / ms_avg, 1),
+            "bitwise_equal_to_plain": bool(same),
+            "launch": {k: info[k] for k in ("threads", "workgroups", "lds_bytes", "tuned")},
+        }
+        del plan, xg, y, t, ys
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +176,9 @@ def main():
     ap.add_argument("--log2n", type=int, default=24)
     ap.add_argument("--levels", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--plain-sample", type=int, default=8192,
+                    help="signals of the batch run through the plain per-signal loop")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -169,7 +262,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "modwt_fwd_group<double, LA8>",
+            "kernel": "modwt_fwd_k<double, LA8, R=1, NT=512, G=10>",
             "achieved": round(fwd_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -179,7 +272,7 @@ def main():
             "avg_launch_ms": round(fwd_ms, 4),
         },
         "inverse_kernel": {
-            "kernel": "modwt_inv_group<double, LA8>",
+            "kernel": "modwt_inv_k<double, LA8> (G=8 R=1, then G=2 R=32)",
             "achieved_GBs": round(inv_gbs, 1),
             "frac": round(inv_gbs / HBM_PEAK_GBS, 4),
             "algorithmic_bytes_per_launch": bi,
@@ -189,6 +282,8 @@ def main():
         "plan": {"fwd": D.modwt_plan(False, n, J, filt), "inv": D.modwt_plan(True, n, J, filt)},
         "reconstruction_relerr": err,
     }
+    if not args.no_batch:
+        out["batched"] = batch_section(max(3, args.steps // 5), 2, args.plain_sample)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(J=J)
     print(json.dumps(out))

@@ -133,6 +133,47 @@ int wh_smooth(void* x, int64_t n, int nlevels, int ttype, int filter, int dtype,
               int threshtype, double* thresh, int hardness, int minlevel, int maxlevel,
               double tol, void* stream);
 
+/* ---- batched smoothing ------------------------------------------------ */
+
+/* Universal-threshold smoothing of many independent signals in one
+   persistent launch.  Replaces the reference's planned list API
+   RgpuTransformList (src/rfunctions.cu:139-154, a stub returning its first
+   argument; R/rfunctions.R:614-643 commented GPUTransformList), and per
+   signal computes exactly what the smoothing entry points do with threshtype
+   UNIV over the full level range (src/rfunctions.cu:82-87: univ_thresh(w,
+   0, 0), then threshold levels 0..J-1, then the inverse; the reference's
+   GPU RgpuSmooth offers MANUAL and CV only, src/rfunctions.cu:101-136).
+
+   Signals are packed back to back in the order given: signal i occupies
+   samples [off_i, off_i + lens[i]) with off_i = sum of the previous
+   lengths.  Per signal: power-of-two length, filter 1..4, ttype 0 (DWT) or
+   1 (MODWT), levels as RgpuSmooth's nlevels (0 = the reference's maximum;
+   the MODWT is fused in one group, so its depth must be <= 8).  hardness is
+   shared by the batch.  A plan owns its device tables, workspace and
+   captured graph; calls on one plan must be ordered (one stream). */
+typedef struct wh_batch wh_batch;
+
+int wh_batch_create(const int* lens, const int* filters, const int* ttypes, const int* levels,
+                    int count, int dtype, int hardness, wh_batch** out);
+void wh_batch_destroy(wh_batch* plan);
+/* x, y: DEVICE packed signals (y may equal x); thresh_out: optional DEVICE
+   array, one threshold per signal. Asynchronous on `stream`. */
+int wh_batch_smooth(wh_batch* plan, const void* x, void* y, double* thresh_out, void* stream);
+/* Time the launch shapes (threads per signal, resident workgroups) on these
+   buffers and keep the fastest in the plan and in a process-wide table keyed
+   by the batch's composition (later plans of the same composition start
+   tuned).  Synchronous; y receives the smoothed signals. */
+int wh_batch_tune(wh_batch* plan, const void* x, void* y, double* thresh_out, void* stream);
+/* Capture wh_batch_smooth on these buffers into a HIP graph held by the
+   plan (replacing an earlier one), and replay it on `stream`. */
+int wh_batch_capture(wh_batch* plan, const void* x, void* y, double* thresh_out);
+int wh_batch_replay(wh_batch* plan, void* stream);
+/* Plan facts: total samples, threads per signal, workgroups, dynamic LDS
+   bytes per workgroup, workspace bytes, whether the shape came from a
+   tuning run (1) or the default rule (0). Any pointer may be NULL. */
+int wh_batch_info(const wh_batch* plan, int64_t* total, int* threads, int* workgroups,
+                  int64_t* lds_bytes, int64_t* workspace_bytes, int* tuned);
+
 #ifdef __cplusplus
 }
 #endif

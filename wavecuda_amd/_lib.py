@@ -65,6 +65,14 @@ SIGNATURES = {
     "RgpuTransform": (None, [_DP, _DP, _IP, _IP, _IP, _IP, _IP, _IP]),
     "RgpuSmooth": (None, [_DP, _IP, _IP, _IP, _IP, _IP, _IP, _DP, _IP, _IP, _IP, _DP]),
     "wh_debug_modwt_plan": (_I, [_I, _I64, _I, _I, _I, C.c_char_p, _I]),
+    "wh_batch_create": (_I, [_IP, _IP, _IP, _IP, _I, _I, _I, C.POINTER(_VP)]),
+    "wh_batch_destroy": (None, [_VP]),
+    "wh_batch_smooth": (_I, [_VP, _VP, _VP, _VP, _VP]),
+    "wh_batch_tune": (_I, [_VP, _VP, _VP, _VP, _VP]),
+    "wh_batch_capture": (_I, [_VP, _VP, _VP, _VP]),
+    "wh_batch_replay": (_I, [_VP, _VP]),
+    "wh_batch_info": (_I, [_VP, C.POINTER(_I64), _IP, _IP, C.POINTER(_I64), C.POINTER(_I64),
+                           _IP]),
 }
 
 _lib = None

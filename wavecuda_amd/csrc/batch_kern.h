@@ -1,0 +1,325 @@
+// batch_kern.h -- device side of the batched smoothing pipeline.
+//
+// Thousands of independent signals of power-of-two lengths are smoothed
+// (forward transform -> hard/soft universal threshold -> inverse) by
+// persistent workgroups that take whole signals from a device work queue:
+// an atomic counter over an item list sorted longest-first, so the large
+// signals start first and the small ones fill the tail.  One launch serves
+// every transform, filter and length of the batch: each item dispatches to
+// a body specialised for its filter (and, for the MODWT, its depth, which is
+// the fused group depth), so the chunk geometry and tap arithmetic are
+// compile-time.  Every workgroup owns a slice of scratch for the
+// coefficients of the signal it is working on; nothing of the pipeline
+// returns to the host.
+//
+//   MODWT: the fused sliding-window forward (modwt_fwd_body) writes the J
+//   levels into scratch, block_mad takes the exact MAD of D_1 (the
+//   reference's univ_thresh(w, 0, 0): src/thresh.cpp:170-201,
+//   src/rfunctions.cu:82-87), and the fused inverse (modwt_inv_body)
+//   thresholds every detail level as it reads it.
+//   DWT: levels run on compact sequences, through scratch while they are
+//   longer than the LDS tile and in LDS afterwards; the MAD is taken over
+//   the level-1 details (all of x[1..n-1] when J = 1, as in the reference)
+//   and the inverse thresholds the details as it reads them.
+#pragma once
+
+#include "blocksel.h"
+#include "dwt_dev.h"
+#include "modwt_kern.h"
+
+namespace wh {
+
+// transform and filter of an item in one word
+__host__ __device__ constexpr int batch_kind(int ttype, int filt) { return (ttype << 3) | filt; }
+__host__ __device__ constexpr int kind_ttype(int kind) { return kind >> 3; }
+__host__ __device__ constexpr int kind_filter(int kind) { return kind & 7; }
+
+// One signal of the batch.
+struct BatchItem {
+  int64_t off;    // first sample in the packed signal buffers
+  double factor;  // sqrt(2 ln n) (sqrt(2 ln(n-1)) for a one-level DWT)
+  int32_t n;      // length (power of two)
+  int32_t J;      // levels
+  int32_t kind;   // batch_kind(ttype, filter)
+  int32_t sig;    // signal index (threshold output slot)
+};
+
+struct BatchArgs {
+  const BatchItem* items;
+  int count;
+  int* counter;  // [0] work-queue head, [1] finished workgroups
+  const void* x;
+  void* y;
+  double* thresh_out;  // optional, one per signal
+  void* scratch;
+  int64_t scratch_stride;  // elements per workgroup
+  int hardness;
+};
+
+__device__ __forceinline__ int batch_next(int* counter, int* sh) {
+  __syncthreads();
+  if (threadIdx.x == 0) *sh = atomicAdd(counter, 1);
+  __syncthreads();
+  return *sh;
+}
+
+// phase boundary inside a workgroup when the phases exchange data through
+// global memory (scratch)
+__device__ __forceinline__ void batch_phase() {
+  __threadfence();
+  __syncthreads();
+}
+
+// MODWT item: fused forward of all J = G levels into scratch (D_j and S_J),
+// MAD of D_1, fused thresholding inverse into y.  Chunk width M = min(2 NT,
+// n); the warm-up covers the coarsest level's reach.
+template <typename T, int FILT, int NT, int G>
+__device__ __forceinline__ void batch_modwt_item(const BatchArgs& b, const BatchItem& I, T* scr,
+                                                 unsigned char* smem) {
+  constexpr int L = Filter<FILT>::L;
+  const int64_t n = I.n;
+  ModwtArgs a{};
+  a.n = n;
+  a.P = n;
+  a.j0 = 0;
+  a.g = G;
+  a.M = int(n < 2 * NT ? n : 2 * NT);
+  const int64_t reach = int64_t(L - 1) * ((int64_t(1) << G) - 1);
+  a.warm = int((reach + a.M - 1) / a.M * a.M);
+  a.nchunks = int((n + a.warm) / a.M);
+  a.in = static_cast<const T*>(b.x) + I.off;
+  a.xmod = scr;
+  modwt_fwd_body<T, FILT, 1, NT, G, 0, kStoreSmooth>(a, 0, 0, smem);
+  batch_phase();
+  const double t = block_mad<T, NT>(scr, n, 1, n, smem) * I.factor;
+  if (threadIdx.x == 0 && b.thresh_out) b.thresh_out[I.sig] = t;
+  a.stop = scr + 2 * n * (G - 1);
+  a.out = static_cast<T*>(b.y) + I.off;
+  a.th_on = 1;
+  a.hardness = b.hardness;
+  a.minlevel = 0;
+  a.maxlevel = G - 1;
+  a.th_value = t;
+  a.th_levels = nullptr;
+  modwt_inv_body<T, FILT, 1, NT, G, 1>(a, 0, 0, smem);
+  batch_phase();
+}
+
+// LDS elements per ping-pong buffer of the DWT tail (64 KB for both types)
+template <typename T>
+constexpr int batch_lds_elems() {
+  return sizeof(T) == 8 ? 4096 : 8192;
+}
+
+// DWT item: levels run on compact sequences, through scratch while longer
+// than the LDS tile and in LDS afterwards; coefficients are kept in the
+// reference's interleaved layout in scratch C (n values, gA/gB n/2 each
+// behind it); MAD of the level-1 details; thresholding inverse into y.
+template <typename T, int FILT, int NT>
+__device__ __forceinline__ void batch_dwt_item(const BatchArgs& b, const BatchItem& I, T* C,
+                                               unsigned char* smem) {
+  constexpr int L = Filter<FILT>::L;
+  constexpr int kBatchLds = batch_lds_elems<T>();
+  T* lA = reinterpret_cast<T*>(smem);
+  T* lB = lA + kBatchLds;
+  const int tid = threadIdx.x;
+  {
+    const int64_t n = I.n;
+    const int J = I.J;
+    T* gA = C + n;          // n/2
+    T* gB = gA + n / 2;     // n/2
+    const T* x = static_cast<const T*>(b.x) + I.off;
+    T* y = static_cast<T*>(b.y) + I.off;
+
+    // ---- forward: global levels while the input is longer than the tile
+    const T* src = x;
+    int64_t N = n;
+    int j = 1;
+    bool toA = true;
+    while (j <= J && N > kBatchLds) {
+      T* dst = toA ? gA : gB;
+      for (int64_t m = tid; m < N / 2; m += NT) {
+        T v[L];
+#pragma unroll
+        for (int k = 0; k < L; ++k) v[k] = src[(2 * m + k) & (N - 1)];
+        T s, d;
+        fwd_pair<T, FILT>(v, s, d);
+        dst[m] = s;
+        C[(2 * m + 1) << (j - 1)] = d;
+      }
+      batch_phase();
+      src = dst;
+      toA = !toA;
+      N >>= 1;
+      ++j;
+    }
+    // remaining levels in LDS (or, when every level ran above, s_J itself)
+    T* cur = lA;
+    T* nxt = lB;
+    const bool all_global = (j > J);
+    if (all_global) {
+      for (int64_t m = tid; m < N; m += NT) C[m << J] = src[m];
+    } else {
+      for (int64_t i = tid; i < N; i += NT) lA[i] = src[i];
+    }
+    __syncthreads();
+    for (; j <= J; ++j) {
+      const int P = int(N / 2);
+      for (int m = tid; m < P; m += NT) {
+        T v[L];
+#pragma unroll
+        for (int k = 0; k < L; ++k) v[k] = cur[(2 * m + k) & (N - 1)];
+        T s, d;
+        fwd_pair<T, FILT>(v, s, d);
+        nxt[m] = s;
+        C[int64_t(2 * m + 1) << (j - 1)] = d;
+      }
+      __syncthreads();
+      T* t = cur;
+      cur = nxt;
+      nxt = t;
+      N = P;
+    }
+    if (!all_global)
+      for (int64_t m = tid; m < N; m += NT) C[m << J] = cur[m];
+    batch_phase();
+
+    // ---- universal threshold from the level-1 details
+    const double t = (J == 1) ? block_mad<T, NT>(C, 1, 1, n - 1, smem) * I.factor
+                              : block_mad<T, NT>(C, 1, 2, n / 2, smem) * I.factor;
+    if (tid == 0 && b.thresh_out) b.thresh_out[I.sig] = t;
+    const T tt = T(t);
+
+    // ---- inverse: levels J.. in LDS while the output fits the tile
+    N = n >> J;  // length of s_J
+    const bool lds_tail = (2 * N <= kBatchLds);
+    if (lds_tail) {
+      for (int64_t m = tid; m < N; m += NT) lA[m] = C[m << J];
+    } else {
+      for (int64_t m = tid; m < N; m += NT) gA[m] = C[m << J];
+    }
+    batch_phase();
+    cur = lA;
+    nxt = lB;
+    j = J;
+    while (j >= 1 && 2 * N <= kBatchLds) {
+      for (int m = tid; m < int(N); m += NT) {
+        T sv[L / 2], dv[L / 2];
+#pragma unroll
+        for (int q = 0; q < L / 2; ++q) {
+          const int mm = (m - q) & int(N - 1);
+          sv[q] = cur[mm];
+          dv[q] = thresh_coef(C[int64_t(2 * mm + 1) << (j - 1)], tt, b.hardness);
+        }
+        T e, o;
+        inv_pair<T, FILT>(sv, dv, e, o);
+        nxt[2 * m] = e;
+        nxt[2 * m + 1] = o;
+      }
+      __syncthreads();
+      T* tp = cur;
+      cur = nxt;
+      nxt = tp;
+      N *= 2;
+      --j;
+    }
+    if (j == 0) {
+      for (int64_t i = tid; i < n; i += NT) y[i] = cur[i];
+    } else {
+      // hand s_j to global (already there when no level ran in LDS) and
+      // finish the long levels there
+      if (lds_tail) {
+        for (int64_t i = tid; i < N; i += NT) gA[i] = cur[i];
+        batch_phase();
+      }
+      const T* s = gA;
+      bool inA = true;
+      for (; j >= 1; --j) {
+        T* out = (j == 1) ? y : (inA ? gB : gA);
+        for (int64_t m = tid; m < N; m += NT) {
+          T sv[L / 2], dv[L / 2];
+#pragma unroll
+          for (int q = 0; q < L / 2; ++q) {
+            const int64_t mm = (m - q) & (N - 1);
+            sv[q] = s[mm];
+            dv[q] = thresh_coef(C[(2 * mm + 1) << (j - 1)], tt, b.hardness);
+          }
+          T e, o;
+          inv_pair<T, FILT>(sv, dv, e, o);
+          out[2 * m] = e;
+          out[2 * m + 1] = o;
+        }
+        batch_phase();
+        s = out;
+        inA = !inA;
+        N *= 2;
+      }
+    }
+  }
+  batch_phase();
+}
+
+template <typename T, int FILT, int NT>
+__device__ __forceinline__ void batch_item(const BatchArgs& b, const BatchItem& I, T* scr,
+                                           unsigned char* smem) {
+  if (kind_ttype(I.kind) == kDWT) {
+    batch_dwt_item<T, FILT, NT>(b, I, scr, smem);
+    return;
+  }
+  switch (I.J) {
+    case 1: batch_modwt_item<T, FILT, NT, 1>(b, I, scr, smem); break;
+    case 2: batch_modwt_item<T, FILT, NT, 2>(b, I, scr, smem); break;
+    case 3: batch_modwt_item<T, FILT, NT, 3>(b, I, scr, smem); break;
+    case 4: batch_modwt_item<T, FILT, NT, 4>(b, I, scr, smem); break;
+    case 5: batch_modwt_item<T, FILT, NT, 5>(b, I, scr, smem); break;
+    case 6: batch_modwt_item<T, FILT, NT, 6>(b, I, scr, smem); break;
+    case 7: batch_modwt_item<T, FILT, NT, 7>(b, I, scr, smem); break;
+    default: batch_modwt_item<T, FILT, NT, 8>(b, I, scr, smem); break;
+  }
+}
+
+// The persistent pipeline: every workgroup takes items until the queue is
+// empty; the last workgroup to leave rewinds the queue, so the launch can be
+// replayed (a captured graph) without a memset node.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void batch_smooth_k(BatchArgs b) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int item_sh;
+  T* scr = static_cast<T*>(b.scratch) + int64_t(blockIdx.x) * b.scratch_stride;
+  for (;;) {
+    const int it = batch_next(b.counter, &item_sh);
+    if (it >= b.count) break;
+    const BatchItem I = b.items[it];
+    switch (kind_filter(I.kind)) {
+      case kHAAR: batch_item<T, kHAAR, NT>(b, I, scr, smem); break;
+      case kD4: batch_item<T, kD4, NT>(b, I, scr, smem); break;
+      case kC6: batch_item<T, kC6, NT>(b, I, scr, smem); break;
+      default: batch_item<T, kLA8, NT>(b, I, scr, smem); break;
+    }
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(b.counter + 1, 1) == int(gridDim.x) - 1) {
+      atomicExch(b.counter, 0);
+      atomicExch(b.counter + 1, 0);
+    }
+  }
+}
+
+// largest MODWT depth of one batch item
+constexpr int kBatchMaxModwtJ = 8;
+
+struct BatchLaunch {
+  int NT;
+  unsigned grid;
+  size_t lds;
+};
+// launch of the batch kernel of element type T, and its resident
+// workgroups per CU for NT threads and lds bytes of dynamic LDS; defined in
+// batch_<type>.hip
+template <typename T>
+int batch_launch(const BatchLaunch& bl, const BatchArgs& a, hipStream_t st);
+template <typename T>
+int batch_occupancy(int NT, size_t lds, int* blocks);
+
+}  // namespace wh

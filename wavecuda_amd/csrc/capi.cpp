@@ -89,6 +89,16 @@ const DeviceInfo& device_info() {
   return di;
 }
 
+void allow_dynamic_lds(const void* kern) {
+  hipFuncAttributes at{};
+  size_t stat = 0;
+  if (hipFuncGetAttributes(&at, kern) == hipSuccess) stat = at.sharedSizeBytes;
+  const size_t maxb = device_info().max_lds_per_block;
+  if (maxb > stat)
+    (void)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(maxb - stat));
+  (void)hipGetLastError();
+}
+
 int scratch_get(size_t bytes, void** out) { return scratch_impl(0, bytes, out); }
 int scratch2_get(size_t bytes, void** out) { return scratch_impl(1, bytes, out); }
 int ws_get(size_t bytes, void** out) { return scratch_impl(2, bytes, out); }

@@ -2,7 +2,6 @@
 // launches.  The kernels and their design notes are in modwt_kern.h; they
 // are instantiated per (element type, filter) in modwt_<type>_<filter>.hip.
 #include <algorithm>
-#include <cstdlib>
 #include <vector>
 
 #include "modwt_kern.h"
@@ -86,10 +85,11 @@ int launch(bool inverse, int filt, const Group& G, const ModwtArgs& a, hipStream
   gl.G = G.g;
   gl.blocks = ((int64_t(1) << G.j0) / G.R) * G.nseg;
   gl.lds = G.lds;
-  // element mapping of the R = 1 group (WH_MODWT_FWD_MAP / WH_MODWT_INV_MAP
-  // override the defaults: adjacent pairs forward, residue-class pairs inverse)
-  const char* ev = std::getenv(inverse ? "WH_MODWT_INV_MAP" : "WH_MODWT_FWD_MAP");
-  gl.map = ev ? std::atoi(ev) : (inverse ? 1 : 0);
+  // element mapping of the R = 1 group: adjacent pairs for the forward
+  // (16-byte LDS and HBM accesses; it is write-bound), two outputs of one
+  // residue class for the inverse (L + 1 interleaved (S, D) reads instead of
+  // 2L; it is LDS-bound)
+  gl.map = inverse ? 1 : 0;
   if (gl.blocks > 0x7fffffff) return set_error(WH_ERR_ARG, "modwt: grid too large");
   switch (filt) {
     case kHAAR: return modwt_launch<T, kHAAR>(inverse, gl, a, st);

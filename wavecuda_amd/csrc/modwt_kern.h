@@ -151,7 +151,22 @@ __device__ __forceinline__ T synthesis(const T* ws, const T* wd, int base) {
 }
 
 // ---------------------------------------------------------------- forward
-template <typename T, int FILT, int R, int G, int MAP>
+// Store policy of the forward: kStoreAll writes S_j and D_j of every level
+// with streaming stores (the transform's output); kStoreSmooth writes only
+// D_j and the last S with ordinary stores (a smoothing pipeline's
+// intermediate, which its inverse re-reads while it may still sit in L2).
+constexpr int kStoreAll = 0;
+constexpr int kStoreSmooth = 1;
+
+template <typename T>
+__device__ __forceinline__ void st_pol(T* p, T v, bool nt) {
+  if (nt)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
+
+template <typename T, int FILT, int R, int G, int MAP, int SP = kStoreAll>
 struct FwdCtx {
   using V = typename Vec2<T>::type;
   static constexpr int L = Filter<FILT>::L;
@@ -208,15 +223,24 @@ struct FwdCtx {
         }
       }
       if constexpr (STORE) {
+        constexpr bool kS = (SP == kStoreAll) || (l == G);
+        constexpr bool kNT = (SP == kStoreAll);
         T* oS = xmod + 2 * n * int64_t(j0 + l - 1) + (c + q0);
         if constexpr (s == 1) {
-          stv_nt(oS, V{s0, s1});
-          stv_nt(oS + n, V{d0, d1});
+          if constexpr (kNT) {
+            stv_nt(oS, V{s0, s1});
+            stv_nt(oS + n, V{d0, d1});
+          } else {
+            if constexpr (kS) stv(oS, V{s0, s1});
+            stv(oS + n, V{d0, d1});
+          }
         } else {
-          __builtin_nontemporal_store(s0, oS);
-          __builtin_nontemporal_store(s1, oS + s);
-          __builtin_nontemporal_store(d0, oS + n);
-          __builtin_nontemporal_store(d1, oS + n + s);
+          if constexpr (kS) {
+            st_pol(oS, s0, kNT);
+            st_pol(oS + s, s1, kNT);
+          }
+          st_pol(oS + n, d0, kNT);
+          st_pol(oS + n + s, d1, kNT);
         }
       }
     } else {
@@ -247,8 +271,13 @@ struct FwdCtx {
       if constexpr (STORE) {
         const int64_t i = (R == 1) ? (c + mm) : r0 + r + ((c + mm) << j0);
         T* oS = xmod + 2 * n * int64_t(j0 + l - 1) + i;
-        stv_nt(oS, V{s0, s1});
-        stv_nt(oS + n, V{d0, d1});
+        if constexpr (SP == kStoreAll) {
+          stv_nt(oS, V{s0, s1});
+          stv_nt(oS + n, V{d0, d1});
+        } else {
+          if constexpr (l == G) stv(oS, V{s0, s1});
+          stv(oS + n, V{d0, d1});
+        }
       }
     }
     __syncthreads();
@@ -272,11 +301,13 @@ struct FwdCtx {
   }
 };
 
-template <typename T, int FILT, int R, int NT, int G, int MAP>
-__global__ __launch_bounds__(NT) void modwt_fwd_k(ModwtArgs a) {
+// One (residue block rb, segment seg) of a forward group, run by the whole
+// workgroup of NT threads; smem holds the rings.
+template <typename T, int FILT, int R, int NT, int G, int MAP, int SP = kStoreAll>
+__device__ __forceinline__ void modwt_fwd_body(const ModwtArgs& a, int64_t rb, int64_t seg,
+                                               unsigned char* smem_raw) {
   using V = typename Vec2<T>::type;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  FwdCtx<T, FILT, R, G, MAP> C;
+  FwdCtx<T, FILT, R, G, MAP, SP> C;
   C.lds = reinterpret_cast<T*>(smem_raw);
   C.xmod = static_cast<T*>(a.xmod);
   C.M = a.M;
@@ -288,8 +319,6 @@ __global__ __launch_bounds__(NT) void modwt_fwd_k(ModwtArgs a) {
   C.mm = e / R;
   C.r = e % R;
   const int64_t Nm = a.n >> a.j0;
-  const int64_t nrb = (int64_t(1) << a.j0) / R;
-  const int64_t rb = int64_t(blockIdx.x) % nrb, seg = int64_t(blockIdx.x) / nrb;
   C.r0 = rb * R;
   const int64_t mhi = (seg + 1) * a.P;
   const T* __restrict__ in = static_cast<const T*>(a.in);
@@ -330,12 +359,21 @@ __global__ __launch_bounds__(NT) void modwt_fwd_k(ModwtArgs a) {
   }
 }
 
+template <typename T, int FILT, int R, int NT, int G, int MAP>
+__global__ __launch_bounds__(NT) void modwt_fwd_k(ModwtArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int64_t nrb = (int64_t(1) << a.j0) / R;
+  modwt_fwd_body<T, FILT, R, NT, G, MAP>(a, int64_t(blockIdx.x) % nrb, int64_t(blockIdx.x) / nrb,
+                                         smem_raw);
+}
+
 // ---------------------------------------------------------------- inverse
-// Ring l (l = 1..G) holds S_l and D_l.  For R = 1 they are interleaved per
-// slot (S, D); for R > 1 the S rings come first and the D rings follow at
-// baseD.
+// Ring l (l = 1..G) holds S_l and D_l: interleaved per slot (S, D) for the
+// residue-class mapping, otherwise the S rings come first and the D rings
+// follow at baseD.
 template <typename T, int FILT, int R, int G, int MAP>
 struct InvCtx {
+  static_assert(R > 1 || MAP == 1, "the R = 1 inverse uses the residue-class mapping");
   using V = typename Vec2<T>::type;
   static constexpr int L = Filter<FILT>::L;
   T* lds;
@@ -365,12 +403,16 @@ struct InvCtx {
     }
     return d;
   }
+  // (S, D) interleaved per slot only for the residue-class mapping, whose
+  // lanes read one position each; the adjacent-pair layouts keep S and D
+  // apart so a 16-byte read returns two consecutive slots of one array
+  static constexpr bool kInter = (R == 1 && MAP == 1);
   __device__ __forceinline__ int sidx(int l, int sl) const {
-    return (R == 1) ? 2 * (inv_off(L, M, l) + sl) : (inv_off(L, M, l) + sl) * R + r;
+    return kInter ? 2 * (inv_off(L, M, l) + sl) : (inv_off(L, M, l) + sl) * R + r;
   }
   __device__ __forceinline__ int didx(int l, int sl) const {
-    return (R == 1) ? 2 * (inv_off(L, M, l) + sl) + 1
-                    : baseD + (inv_off(L, M, l) + sl) * R + r;
+    return kInter ? 2 * (inv_off(L, M, l) + sl) + 1
+                  : baseD + (inv_off(L, M, l) + sl) * R + r;
   }
 
   // HBM loads of chunk cc for every ring
@@ -388,7 +430,7 @@ struct InvCtx {
       int so = head[l] + inv_win(L, l) + mm;
       if (so >= Q) so -= Q;
       const V d = thr2(pre[l], thr(l));
-      if constexpr (R == 1) {
+      if constexpr (kInter) {
         // slots so, so+1 (so even, Q even: no wrap between them)
         T* p = lds + sidx(l, so);
         if (l == G) {
@@ -440,20 +482,11 @@ struct InvCtx {
       for (int kk = 0; kk < L; ++kk) {
         int sk = sl - kk * s;
         if (sk < 0) sk += Q;
-        if constexpr (R == 1) {
-          // adjacent positions: interleaved (S, D) of slots sk and sk + 1
-          V p0 = ldv(lds + sidx(l, sk)), p1 = ldv(lds + sidx(l, sk) + 2);
-          ws0[L - 1 - kk] = p0.x;
-          wd0[L - 1 - kk] = p0.y;
-          ws1[L - 1 - kk] = p1.x;
-          wd1[L - 1 - kk] = p1.y;
-        } else {
-          V vs = ldv(lds + sidx(l, sk)), vd = ldv(lds + didx(l, sk));
-          ws0[L - 1 - kk] = vs.x;
-          ws1[L - 1 - kk] = vs.y;
-          wd0[L - 1 - kk] = vd.x;
-          wd1[L - 1 - kk] = vd.y;
-        }
+        V vs = ldv(lds + sidx(l, sk)), vd = ldv(lds + didx(l, sk));
+        ws0[L - 1 - kk] = vs.x;
+        ws1[L - 1 - kk] = vs.y;
+        wd0[L - 1 - kk] = vd.x;
+        wd1[L - 1 - kk] = vd.y;
       }
       e0 = synthesis<T, FILT>(ws0, wd0, L - 1);
       e1 = synthesis<T, FILT>(ws1, wd1, L - 1);
@@ -468,10 +501,6 @@ struct InvCtx {
         so += s;
         if (so >= Qo) so -= Qo;
         lds[sidx(l - 1, so)] = e1;
-      } else if constexpr (R == 1) {
-        // adjacent pair: slots so, so+1 of the interleaved ring
-        lds[sidx(l - 1, so)] = e0;
-        lds[sidx(l - 1, so) + 2] = e1;
       } else {
         stv(lds + sidx(l - 1, so), V{e0, e1});
       }
@@ -502,9 +531,9 @@ struct InvCtx {
 };
 
 template <typename T, int FILT, int R, int NT, int G, int MAP>
-__global__ __launch_bounds__(NT) void modwt_inv_k(ModwtArgs a) {
+__device__ __forceinline__ void modwt_inv_body(const ModwtArgs& a, int64_t rb, int64_t seg,
+                                               unsigned char* smem_raw) {
   constexpr int L = Filter<FILT>::L;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
   InvCtx<T, FILT, R, G, MAP> C;
   C.lds = reinterpret_cast<T*>(smem_raw);
   C.a = &a;
@@ -515,12 +544,10 @@ __global__ __launch_bounds__(NT) void modwt_inv_k(ModwtArgs a) {
   C.j0 = a.j0;
   C.n = a.n;
   C.Nm = a.n >> a.j0;
-  C.baseD = (R == 1) ? 0 : inv_off(L, a.M, G + 1) * R;
+  C.baseD = (R == 1 && MAP == 1) ? 0 : inv_off(L, a.M, G + 1) * R;
   const int e = (2 * int(threadIdx.x)) % (a.M * R);
   C.mm = e / R;
   C.r = e % R;
-  const int64_t nrb = (int64_t(1) << a.j0) / R;
-  const int64_t rb = int64_t(blockIdx.x) % nrb, seg = int64_t(blockIdx.x) / nrb;
   C.r0 = rb * R;
   const int64_t cbot = seg * a.P - a.warm;
 #pragma unroll
@@ -551,10 +578,18 @@ __global__ __launch_bounds__(NT) void modwt_inv_k(ModwtArgs a) {
   }
 }
 
+template <typename T, int FILT, int R, int NT, int G, int MAP>
+__global__ __launch_bounds__(NT) void modwt_inv_k(ModwtArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int64_t nrb = (int64_t(1) << a.j0) / R;
+  modwt_inv_body<T, FILT, R, NT, G, MAP>(a, int64_t(blockIdx.x) % nrb, int64_t(blockIdx.x) / nrb,
+                                         smem_raw);
+}
+
 // Launch of a (T, FILT) group kernel; defined by instantiation files.
 struct GroupLaunch {
   int R, NT, G;
-  int map;  // R = 1 element mapping: 0 adjacent pairs, 1 two outputs of one residue class
+  int map;  // R = 1 element mapping: 0 adjacent pairs, 1 two outputs of one residue class (set by the planner)
   int64_t blocks;
   size_t lds;
 };

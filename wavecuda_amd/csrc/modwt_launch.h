@@ -12,33 +12,24 @@ namespace wh {
 template <typename KERN>
 inline void modwt_allow_lds(KERN kern) {
   static std::once_flag once;
-  std::call_once(once, [&] {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  });
+  std::call_once(once, [&] { allow_dynamic_lds(reinterpret_cast<const void*>(kern)); });
 }
 
-template <typename T, int FILT, int R, int NT, int G, int MAP>
-int modwt_launch_map(bool inverse, const GroupLaunch& gl, const ModwtArgs& a, hipStream_t st) {
+template <typename T, int FILT, int R, int NT, int G>
+int modwt_launch_one(bool inverse, const GroupLaunch& gl, const ModwtArgs& a, hipStream_t st) {
+  (void)gl;
   if (inverse) {
+    constexpr int MAP = (R == 1) ? 1 : 0;
     auto kern = modwt_inv_k<T, FILT, R, NT, G, MAP>;
     modwt_allow_lds(kern);
     hipLaunchKernelGGL(kern, dim3(unsigned(gl.blocks)), dim3(NT), gl.lds, st, a);
   } else {
-    auto kern = modwt_fwd_k<T, FILT, R, NT, G, MAP>;
+    auto kern = modwt_fwd_k<T, FILT, R, NT, G, 0>;
     modwt_allow_lds(kern);
     hipLaunchKernelGGL(kern, dim3(unsigned(gl.blocks)), dim3(NT), gl.lds, st, a);
   }
   WH_HIP_CHECK(hipGetLastError());
   return WH_OK;
-}
-
-template <typename T, int FILT, int R, int NT, int G>
-int modwt_launch_one(bool inverse, const GroupLaunch& gl, const ModwtArgs& a, hipStream_t st) {
-  if constexpr (R == 1) {
-    if (gl.map == 1) return modwt_launch_map<T, FILT, R, NT, G, 1>(inverse, gl, a, st);
-  }
-  return modwt_launch_map<T, FILT, R, NT, G, 0>(inverse, gl, a, st);
 }
 
 template <typename T, int FILT, int R, int NT, int G>

@@ -33,6 +33,11 @@ int scratch2_get(size_t bytes, void** out);
 int ws_get(size_t bytes, void** out);
 int stats_get(size_t bytes, void** out);
 
+// Let kernel `kern` take all of the device's LDS beyond its static LDS as
+// dynamic LDS.  A refused request is cleared from HIP's last error so it
+// cannot surface in a later, unrelated call.
+void allow_dynamic_lds(const void* kern);
+
 // transform back ends (device pointers)
 int modwt_forward(const void* x, void* xmod, int64_t n, int J, int filt, int dtype,
                   hipStream_t st);
