@@ -106,6 +106,46 @@ int wh_dwt_inv(const void* y, void* x, int64_t n, int nlevels, int filter, int d
 int wh_threshold(void* coeffs, int64_t n, int nlevels, int ttype, int filter, int dtype,
                  const wh_thresh* thr, void* stream);
 
+/* ---- many signals, adjoints ------------------------------------------ */
+
+/* The MODWT of `count` signals of n samples at once (1 <= count <= 65535):
+   x and the inverse's output hold the signals n apart, xmod holds their
+   coefficient blocks 2 n J apart.  Same arithmetic per signal as
+   wh_modwt_fwd / wh_modwt_inv (reference src/daub4.cpp:510-589). */
+int wh_modwt_fwd_many(const void* x, void* xmod, int64_t n, int64_t count, int nlevels,
+                      int filter, int dtype, void* stream);
+int wh_modwt_inv_many(const void* xmod, void* x, int64_t n, int64_t count, int nlevels,
+                      int filter, int dtype, const wh_thresh* thr, void* stream);
+/* Adjoints (vector-Jacobian products) for differentiation; no counterpart
+   in the reference, derived from the transforms above.
+   wh_modwt_fwd_adj: cotangent of xmod (every S_j and D_j) -> cotangent of x.
+   wh_modwt_inv_adj: cotangent of x -> cotangent of xmod (zero in the S_j
+   slots, j < J, which the inverse does not read).
+   wh_dwt_fwd_adj / wh_dwt_inv_adj: the DWT is orthogonal; these run the
+   inverse / forward kernels on the cotangent. */
+int wh_modwt_fwd_adj(const void* gxmod, void* gx, int64_t n, int64_t count, int nlevels,
+                     int filter, int dtype, void* stream);
+int wh_modwt_inv_adj(const void* gx, void* gxmod, int64_t n, int64_t count, int nlevels,
+                     int filter, int dtype, void* stream);
+int wh_dwt_fwd_adj(const void* gy, void* gx, int64_t n, int nlevels, int filter, int dtype,
+                   void* stream);
+int wh_dwt_inv_adj(const void* gx, void* gy, int64_t n, int nlevels, int filter, int dtype,
+                   void* stream);
+/* wh_threshold over `count` transformed signals (DWT n apart, MODWT 2nJ
+   apart).  reference src/thresh.cpp:15-83, :142-157. */
+int wh_threshold_many(void* coeffs, int64_t n, int64_t count, int nlevels, int ttype, int filter,
+                      int dtype, const wh_thresh* thr, void* stream);
+/* Gradient of the thresholding y = threshold(c): given the coefficients c
+   before thresholding and the cotangent gy of y, writes gc (the cotangent
+   of c; gc may equal gy) and, when glam is not NULL, glam[l] (DEVICE, one
+   double per level, J entries) = sum over all signals of dL/dt_l: for the
+   soft rule -sign(c) gy over |c| > t_l, for the hard rule 0 (almost
+   everywhere); levels outside [minlevel, maxlevel] get 0.  At |c| = t the
+   soft derivative is taken as 0 (the rule maps it to 0). */
+int wh_thresh_grad(const void* c, const void* gy, void* gc, double* glam, int64_t n,
+                   int64_t count, int nlevels, int ttype, int filter, int dtype,
+                   const wh_thresh* thr, void* stream);
+
 /* ---- statistics ------------------------------------------------------- */
 
 /* Exact median / MAD (1.4826 median|x - median x|) of `count` values,

@@ -65,6 +65,15 @@ SIGNATURES = {
     "RgpuTransform": (None, [_DP, _DP, _IP, _IP, _IP, _IP, _IP, _IP]),
     "RgpuSmooth": (None, [_DP, _IP, _IP, _IP, _IP, _IP, _IP, _DP, _IP, _IP, _IP, _DP]),
     "wh_debug_modwt_plan": (_I, [_I, _I64, _I, _I, _I, C.c_char_p, _I]),
+    "wh_modwt_fwd_many": (_I, [_VP, _VP, _I64, _I64, _I, _I, _I, _VP]),
+    "wh_modwt_inv_many": (_I, [_VP, _VP, _I64, _I64, _I, _I, _I, C.POINTER(wh_thresh), _VP]),
+    "wh_modwt_fwd_adj": (_I, [_VP, _VP, _I64, _I64, _I, _I, _I, _VP]),
+    "wh_modwt_inv_adj": (_I, [_VP, _VP, _I64, _I64, _I, _I, _I, _VP]),
+    "wh_dwt_fwd_adj": (_I, [_VP, _VP, _I64, _I, _I, _I, _VP]),
+    "wh_dwt_inv_adj": (_I, [_VP, _VP, _I64, _I, _I, _I, _VP]),
+    "wh_threshold_many": (_I, [_VP, _I64, _I64, _I, _I, _I, _I, C.POINTER(wh_thresh), _VP]),
+    "wh_thresh_grad": (_I, [_VP, _VP, _VP, _VP, _I64, _I64, _I, _I, _I, _I,
+                            C.POINTER(wh_thresh), _VP]),
     "wh_batch_create": (_I, [_IP, _IP, _IP, _IP, _I, _I, _I, C.POINTER(_VP)]),
     "wh_batch_destroy": (None, [_VP]),
     "wh_batch_smooth": (_I, [_VP, _VP, _VP, _VP, _VP]),

@@ -164,7 +164,7 @@ int wh_modwt_fwd(const void* x, void* xmod, int64_t n, int nlevels, int filter, 
                  void* stream) {
   int J, rc = check_common(x, xmod, n, nlevels, filter, dtype, &J);
   if (rc) return rc;
-  return modwt_forward(x, xmod, n, J, filter, dtype, static_cast<hipStream_t>(stream));
+  return modwt_forward(x, xmod, n, 1, J, filter, dtype, static_cast<hipStream_t>(stream));
 }
 
 int wh_modwt_inv(const void* xmod, void* x, int64_t n, int nlevels, int filter, int dtype,
@@ -172,7 +172,7 @@ int wh_modwt_inv(const void* xmod, void* x, int64_t n, int nlevels, int filter, 
   int J, rc = check_common(xmod, x, n, nlevels, filter, dtype, &J);
   if (rc) return rc;
   if ((rc = check_thresh(thr, J))) return rc;
-  return modwt_inverse(xmod, x, n, J, filter, dtype, thr, static_cast<hipStream_t>(stream));
+  return modwt_inverse(xmod, x, n, 1, J, filter, dtype, thr, static_cast<hipStream_t>(stream));
 }
 
 int wh_dwt_fwd(const void* x, void* y, int64_t n, int nlevels, int filter, int dtype,
@@ -197,7 +197,81 @@ int wh_threshold(void* coeffs, int64_t n, int nlevels, int ttype, int filter, in
   if (ttype != kDWT && ttype != kMODWT) return set_error(WH_ERR_ARG, "ttype must be 0 or 1");
   if (!thr || !thr->enabled) return WH_OK;
   if ((rc = check_thresh(thr, J))) return rc;
-  return threshold_inplace(coeffs, n, J, ttype, dtype, thr, static_cast<hipStream_t>(stream));
+  return threshold_inplace(coeffs, n, 1, J, ttype, dtype, thr, static_cast<hipStream_t>(stream));
+}
+
+static int check_count(int64_t count) {
+  if (count < 1 || count > 65535) return set_error(WH_ERR_ARG, "count must be 1..65535");
+  return WH_OK;
+}
+
+int wh_modwt_fwd_many(const void* x, void* xmod, int64_t n, int64_t count, int nlevels,
+                      int filter, int dtype, void* stream) {
+  int J, rc = check_common(x, xmod, n, nlevels, filter, dtype, &J);
+  if (rc || (rc = check_count(count))) return rc;
+  return modwt_forward(x, xmod, n, count, J, filter, dtype, static_cast<hipStream_t>(stream));
+}
+
+int wh_modwt_inv_many(const void* xmod, void* x, int64_t n, int64_t count, int nlevels,
+                      int filter, int dtype, const wh_thresh* thr, void* stream) {
+  int J, rc = check_common(xmod, x, n, nlevels, filter, dtype, &J);
+  if (rc || (rc = check_count(count))) return rc;
+  if ((rc = check_thresh(thr, J))) return rc;
+  return modwt_inverse(xmod, x, n, count, J, filter, dtype, thr, static_cast<hipStream_t>(stream));
+}
+
+int wh_modwt_fwd_adj(const void* gxmod, void* gx, int64_t n, int64_t count, int nlevels,
+                     int filter, int dtype, void* stream) {
+  int J, rc = check_common(gxmod, gx, n, nlevels, filter, dtype, &J);
+  if (rc || (rc = check_count(count))) return rc;
+  return modwt_forward_adjoint(gxmod, gx, n, count, J, filter, dtype,
+                               static_cast<hipStream_t>(stream));
+}
+
+int wh_modwt_inv_adj(const void* gx, void* gxmod, int64_t n, int64_t count, int nlevels,
+                     int filter, int dtype, void* stream) {
+  int J, rc = check_common(gx, gxmod, n, nlevels, filter, dtype, &J);
+  if (rc || (rc = check_count(count))) return rc;
+  return modwt_inverse_adjoint(gx, gxmod, n, count, J, filter, dtype,
+                               static_cast<hipStream_t>(stream));
+}
+
+int wh_dwt_fwd_adj(const void* gy, void* gx, int64_t n, int nlevels, int filter, int dtype,
+                   void* stream) {
+  int J, rc = check_common(gy, gx, n, nlevels, filter, dtype, &J);
+  if (rc) return rc;
+  return dwt_inverse(gy, gx, n, J, filter, dtype, nullptr, static_cast<hipStream_t>(stream));
+}
+
+int wh_dwt_inv_adj(const void* gx, void* gy, int64_t n, int nlevels, int filter, int dtype,
+                   void* stream) {
+  int J, rc = check_common(gx, gy, n, nlevels, filter, dtype, &J);
+  if (rc) return rc;
+  return dwt_forward(gx, gy, n, J, filter, dtype, static_cast<hipStream_t>(stream));
+}
+
+int wh_threshold_many(void* coeffs, int64_t n, int64_t count, int nlevels, int ttype, int filter,
+                      int dtype, const wh_thresh* thr, void* stream) {
+  int J, rc = check_common(coeffs, coeffs, n, nlevels, filter, dtype, &J);
+  if (rc || (rc = check_count(count))) return rc;
+  if (ttype != kDWT && ttype != kMODWT) return set_error(WH_ERR_ARG, "ttype must be 0 or 1");
+  if (!thr || !thr->enabled) return WH_OK;
+  if ((rc = check_thresh(thr, J))) return rc;
+  return threshold_inplace(coeffs, n, count, J, ttype, dtype, thr,
+                           static_cast<hipStream_t>(stream));
+}
+
+int wh_thresh_grad(const void* c, const void* gy, void* gc, double* glam, int64_t n,
+                   int64_t count, int nlevels, int ttype, int filter, int dtype,
+                   const wh_thresh* thr, void* stream) {
+  int J, rc = check_common(c, gy, n, nlevels, filter, dtype, &J);
+  if (rc || (rc = check_count(count))) return rc;
+  if (!gc) return set_error(WH_ERR_ARG, "null pointer");
+  if (ttype != kDWT && ttype != kMODWT) return set_error(WH_ERR_ARG, "ttype must be 0 or 1");
+  if (!thr || !thr->enabled) return set_error(WH_ERR_ARG, "threshold gradient needs a threshold");
+  if ((rc = check_thresh(thr, J))) return rc;
+  return thresh_grad(c, gy, gc, glam, n, count, J, ttype, dtype, thr,
+                     static_cast<hipStream_t>(stream));
 }
 
 static int stats_state(SelState** st) {

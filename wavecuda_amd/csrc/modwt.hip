@@ -78,13 +78,17 @@ std::vector<Group> plan_groups(bool inverse, int64_t n, int J, int L, size_t esz
 }
 
 template <typename T>
-int launch(bool inverse, int filt, const Group& G, const ModwtArgs& a, hipStream_t st) {
+int launch(bool inverse, int filt, const Group& G, const ModwtArgs& a, int64_t count, int sp,
+           hipStream_t st) {
   GroupLaunch gl;
   gl.R = G.R;
   gl.NT = G.NT;
   gl.G = G.g;
+  gl.sp = sp;
+  gl.count = count;
   gl.blocks = ((int64_t(1) << G.j0) / G.R) * G.nseg;
   gl.lds = G.lds;
+  if (count < 1 || count > 65535) return set_error(WH_ERR_ARG, "modwt: signal count out of range");
   // element mapping of the R = 1 group: adjacent pairs for the forward
   // (16-byte LDS and HBM accesses; it is write-bound), two outputs of one
   // residue class for the inverse (L + 1 interleaved (S, D) reads instead of
@@ -112,40 +116,70 @@ ModwtArgs args_of(const Group& G, int64_t n) {
 }
 
 template <typename T>
-int modwt_fwd_t(const void* x, void* xmod, int64_t n, int J, int filt, hipStream_t st) {
+__global__ void zero_slot_k(T* __restrict__ p, int64_t n, int64_t stride) {
+  T* q = p + int64_t(blockIdx.y) * stride;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    q[i] = T(0);
+}
+
+template <typename T>
+int modwt_fwd_t(const void* x, void* xmod, int64_t n, int64_t count, int J, int filt, int sp,
+                hipStream_t st) {
   const int L = filter_length(filt);
   auto groups = plan_groups(false, n, J, L, sizeof(T));
+  const int64_t per = 2 * n * J;
   for (const Group& G : groups) {
     ModwtArgs a = args_of(G, n);
     a.in = (G.j0 == 0) ? x : static_cast<const T*>(xmod) + 2 * n * int64_t(G.j0 - 1);
+    a.bs_in = (G.j0 == 0) ? n : per;
     a.xmod = xmod;
-    int rc = launch<T>(false, filt, G, a, st);
+    a.bs_xmod = per;
+    int rc = launch<T>(false, filt, G, a, count, sp, st);
     if (rc) return rc;
+  }
+  if (sp == kStoreAdjoint) {
+    // the S slots at group boundaries carried the next group's input; the
+    // adjoint of the inverse is zero there
+    for (size_t gi = 0; gi + 1 < groups.size(); ++gi) {
+      const int j = groups[gi].j0 + groups[gi].g;  // 1-based level of the slot
+      T* slot = static_cast<T*>(xmod) + 2 * n * int64_t(j - 1);
+      const unsigned blocks = unsigned(std::min<int64_t>((n + 255) / 256, 1024));
+      hipLaunchKernelGGL(zero_slot_k<T>, dim3(blocks, unsigned(count)), dim3(256), 0, st, slot,
+                         n, per);
+      WH_HIP_CHECK(hipGetLastError());
+    }
   }
   return WH_OK;
 }
 
 template <typename T>
-int modwt_inv_t(const void* xmod, void* x, int64_t n, int J, int filt, const wh_thresh* thr,
-                hipStream_t st) {
+int modwt_inv_t(const void* xmod, void* x, int64_t n, int64_t count, int J, int filt,
+                const wh_thresh* thr, hipStream_t st) {
   const int L = filter_length(filt);
   auto groups = plan_groups(true, n, J, L, sizeof(T));
-  // the S_{j0} handed between groups lives in library scratch (2 n values)
+  const int64_t per = 2 * n * J;
+  // the S_{j0} handed between groups lives in library scratch (2 n values
+  // per signal: two halves used alternately)
   T* scratch = nullptr;
   if (groups.size() > 1) {
     void* p = nullptr;
-    int rc = scratch_get(2 * size_t(n) * sizeof(T), &p);
+    int rc = scratch_get(2 * size_t(n) * size_t(count) * sizeof(T), &p);
     if (rc) return rc;
     scratch = static_cast<T*>(p);
   }
   const T* stop = static_cast<const T*>(xmod) + 2 * n * int64_t(J - 1);
+  int64_t bs_stop = per;
   for (int gi = int(groups.size()) - 1; gi >= 0; --gi) {
     const Group& G = groups[gi];
     ModwtArgs a = args_of(G, n);
     a.xmod = const_cast<void*>(xmod);
+    a.bs_xmod = per;
     a.stop = stop;
-    T* dst = (G.j0 == 0) ? static_cast<T*>(x) : scratch + (gi & 1) * n;
+    a.bs_stop = bs_stop;
+    T* dst = (G.j0 == 0) ? static_cast<T*>(x) : scratch + (gi & 1) * n * count;
     a.out = dst;
+    a.bs_out = n;
     if (thr && thr->enabled) {
       a.th_on = 1;
       a.hardness = thr->hardness;
@@ -154,25 +188,32 @@ int modwt_inv_t(const void* xmod, void* x, int64_t n, int J, int filt, const wh_
       a.th_value = thr->value;
       a.th_levels = thr->levels;
     }
-    int rc = launch<T>(true, filt, G, a, st);
+    int rc = launch<T>(true, filt, G, a, count, kStoreAll, st);
     if (rc) return rc;
     stop = dst;
+    bs_stop = n;
   }
   return WH_OK;
 }
 
 }  // namespace
 
-int modwt_forward(const void* x, void* xmod, int64_t n, int J, int filt, int dtype,
-                  hipStream_t st) {
-  return dtype == WH_F32 ? modwt_fwd_t<float>(x, xmod, n, J, filt, st)
-                         : modwt_fwd_t<double>(x, xmod, n, J, filt, st);
+int modwt_forward(const void* x, void* xmod, int64_t n, int64_t count, int J, int filt,
+                  int dtype, hipStream_t st) {
+  return dtype == WH_F32 ? modwt_fwd_t<float>(x, xmod, n, count, J, filt, kStoreAll, st)
+                         : modwt_fwd_t<double>(x, xmod, n, count, J, filt, kStoreAll, st);
 }
 
-int modwt_inverse(const void* xmod, void* x, int64_t n, int J, int filt, int dtype,
+int modwt_inverse(const void* xmod, void* x, int64_t n, int64_t count, int J, int filt, int dtype,
                   const wh_thresh* thr, hipStream_t st) {
-  return dtype == WH_F32 ? modwt_inv_t<float>(xmod, x, n, J, filt, thr, st)
-                         : modwt_inv_t<double>(xmod, x, n, J, filt, thr, st);
+  return dtype == WH_F32 ? modwt_inv_t<float>(xmod, x, n, count, J, filt, thr, st)
+                         : modwt_inv_t<double>(xmod, x, n, count, J, filt, thr, st);
+}
+
+int modwt_inverse_adjoint(const void* gx, void* gxmod, int64_t n, int64_t count, int J, int filt,
+                          int dtype, hipStream_t st) {
+  return dtype == WH_F32 ? modwt_fwd_t<float>(gx, gxmod, n, count, J, filt, kStoreAdjoint, st)
+                         : modwt_fwd_t<double>(gx, gxmod, n, count, J, filt, kStoreAdjoint, st);
 }
 
 std::string modwt_plan_string(bool inverse, int64_t n, int J, int filt, int dtype) {

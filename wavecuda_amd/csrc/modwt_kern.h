@@ -54,7 +54,22 @@ struct ModwtArgs {
   int th_on, hardness, minlevel, maxlevel;
   double th_value;
   const double* th_levels;
+  // signal batches: blockIdx.y selects the signal, offset by these strides
+  // (elements) from in / xmod / stop / out
+  int64_t bs_in, bs_xmod, bs_stop, bs_out;
 };
+
+// the arguments of the signal handled by this workgroup (blockIdx.y)
+template <typename T>
+__device__ __forceinline__ ModwtArgs batch_slice(const ModwtArgs& a) {
+  ModwtArgs b = a;
+  const int64_t y = blockIdx.y;
+  b.in = static_cast<const T*>(a.in) + y * a.bs_in;
+  b.xmod = static_cast<T*>(a.xmod) + y * a.bs_xmod;
+  b.stop = static_cast<const T*>(a.stop) + y * a.bs_stop;
+  b.out = static_cast<T*>(a.out) + y * a.bs_out;
+  return b;
+}
 
 constexpr int kGmax = 10;
 
@@ -154,9 +169,14 @@ __device__ __forceinline__ T synthesis(const T* ws, const T* wd, int base) {
 // Store policy of the forward: kStoreAll writes S_j and D_j of every level
 // with streaming stores (the transform's output); kStoreSmooth writes only
 // D_j and the last S with ordinary stores (a smoothing pipeline's
-// intermediate, which its inverse re-reads while it may still sit in L2).
+// intermediate, which its inverse re-reads while it may still sit in L2);
+// kStoreAdjoint makes the forward the adjoint of the inverse: level l of the
+// group stores 2^-l D and, at the group's last level, 2^-G S, and zeros
+// in the other S slots (the inverse never reads them).  The scalings are
+// powers of two, hence exact.
 constexpr int kStoreAll = 0;
 constexpr int kStoreSmooth = 1;
+constexpr int kStoreAdjoint = 2;
 
 template <typename T>
 __device__ __forceinline__ void st_pol(T* p, T v, bool nt) {
@@ -222,7 +242,20 @@ struct FwdCtx {
           ro[so] = s1;
         }
       }
-      if constexpr (STORE) {
+      if constexpr (STORE && SP == kStoreAdjoint) {
+        constexpr T sc = T(1.0 / double(1 << l));
+        const T ks = (l == G) ? sc : T(0);
+        T* oS = xmod + 2 * n * int64_t(j0 + l - 1) + (c + q0);
+        if constexpr (s == 1) {
+          stv_nt(oS, V{s0 * ks, s1 * ks});
+          stv_nt(oS + n, V{d0 * sc, d1 * sc});
+        } else {
+          __builtin_nontemporal_store(s0 * ks, oS);
+          __builtin_nontemporal_store(s1 * ks, oS + s);
+          __builtin_nontemporal_store(d0 * sc, oS + n);
+          __builtin_nontemporal_store(d1 * sc, oS + n + s);
+        }
+      } else if constexpr (STORE) {
         constexpr bool kS = (SP == kStoreAll) || (l == G);
         constexpr bool kNT = (SP == kStoreAll);
         T* oS = xmod + 2 * n * int64_t(j0 + l - 1) + (c + q0);
@@ -274,6 +307,11 @@ struct FwdCtx {
         if constexpr (SP == kStoreAll) {
           stv_nt(oS, V{s0, s1});
           stv_nt(oS + n, V{d0, d1});
+        } else if constexpr (SP == kStoreAdjoint) {
+          constexpr T sc = T(1.0 / double(1 << l));
+          const T ks = (l == G) ? sc : T(0);
+          stv_nt(oS, V{s0 * ks, s1 * ks});
+          stv_nt(oS + n, V{d0 * sc, d1 * sc});
         } else {
           if constexpr (l == G) stv(oS, V{s0, s1});
           stv(oS + n, V{d0, d1});
@@ -359,12 +397,13 @@ __device__ __forceinline__ void modwt_fwd_body(const ModwtArgs& a, int64_t rb, i
   }
 }
 
-template <typename T, int FILT, int R, int NT, int G, int MAP>
+template <typename T, int FILT, int R, int NT, int G, int MAP, int SP>
 __global__ __launch_bounds__(NT) void modwt_fwd_k(ModwtArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int64_t nrb = (int64_t(1) << a.j0) / R;
-  modwt_fwd_body<T, FILT, R, NT, G, MAP>(a, int64_t(blockIdx.x) % nrb, int64_t(blockIdx.x) / nrb,
-                                         smem_raw);
+  const ModwtArgs b = batch_slice<T>(a);
+  modwt_fwd_body<T, FILT, R, NT, G, MAP, SP>(b, int64_t(blockIdx.x) % nrb,
+                                             int64_t(blockIdx.x) / nrb, smem_raw);
 }
 
 // ---------------------------------------------------------------- inverse
@@ -582,7 +621,8 @@ template <typename T, int FILT, int R, int NT, int G, int MAP>
 __global__ __launch_bounds__(NT) void modwt_inv_k(ModwtArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int64_t nrb = (int64_t(1) << a.j0) / R;
-  modwt_inv_body<T, FILT, R, NT, G, MAP>(a, int64_t(blockIdx.x) % nrb, int64_t(blockIdx.x) / nrb,
+  const ModwtArgs b = batch_slice<T>(a);
+  modwt_inv_body<T, FILT, R, NT, G, MAP>(b, int64_t(blockIdx.x) % nrb, int64_t(blockIdx.x) / nrb,
                                          smem_raw);
 }
 
@@ -590,7 +630,9 @@ __global__ __launch_bounds__(NT) void modwt_inv_k(ModwtArgs a) {
 struct GroupLaunch {
   int R, NT, G;
   int map;  // R = 1 element mapping: 0 adjacent pairs, 1 two outputs of one residue class (set by the planner)
+  int sp;   // forward store policy (kStoreAll or kStoreAdjoint)
   int64_t blocks;
+  int64_t count;  // signals (grid y)
   size_t lds;
 };
 template <typename T, int FILT>

@@ -17,16 +17,20 @@ inline void modwt_allow_lds(KERN kern) {
 
 template <typename T, int FILT, int R, int NT, int G>
 int modwt_launch_one(bool inverse, const GroupLaunch& gl, const ModwtArgs& a, hipStream_t st) {
-  (void)gl;
+  const dim3 grid(unsigned(gl.blocks), unsigned(gl.count));
   if (inverse) {
     constexpr int MAP = (R == 1) ? 1 : 0;
     auto kern = modwt_inv_k<T, FILT, R, NT, G, MAP>;
     modwt_allow_lds(kern);
-    hipLaunchKernelGGL(kern, dim3(unsigned(gl.blocks)), dim3(NT), gl.lds, st, a);
-  } else {
-    auto kern = modwt_fwd_k<T, FILT, R, NT, G, 0>;
+    hipLaunchKernelGGL(kern, grid, dim3(NT), gl.lds, st, a);
+  } else if (gl.sp == kStoreAdjoint) {
+    auto kern = modwt_fwd_k<T, FILT, R, NT, G, 0, kStoreAdjoint>;
     modwt_allow_lds(kern);
-    hipLaunchKernelGGL(kern, dim3(unsigned(gl.blocks)), dim3(NT), gl.lds, st, a);
+    hipLaunchKernelGGL(kern, grid, dim3(NT), gl.lds, st, a);
+  } else {
+    auto kern = modwt_fwd_k<T, FILT, R, NT, G, 0, kStoreAll>;
+    modwt_allow_lds(kern);
+    hipLaunchKernelGGL(kern, grid, dim3(NT), gl.lds, st, a);
   }
   WH_HIP_CHECK(hipGetLastError());
   return WH_OK;

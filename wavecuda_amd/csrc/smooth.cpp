@@ -43,13 +43,13 @@ int64_t coeff_len(int ttype, int64_t n, int J) { return ttype == kDWT ? n : 2 * 
 int fwd(int ttype, const void* x, void* w, int64_t n, int J, int filt, int dtype,
         hipStream_t st) {
   return ttype == kDWT ? dwt_forward(x, w, n, J, filt, dtype, st)
-                       : modwt_forward(x, w, n, J, filt, dtype, st);
+                       : modwt_forward(x, w, n, 1, J, filt, dtype, st);
 }
 
 int inv(int ttype, const void* w, void* x, int64_t n, int J, int filt, int dtype,
         const wh_thresh* thr, hipStream_t st) {
   return ttype == kDWT ? dwt_inverse(w, x, n, J, filt, dtype, thr, st)
-                       : modwt_inverse(w, x, n, J, filt, dtype, thr, st);
+                       : modwt_inverse(w, x, n, 1, J, filt, dtype, thr, st);
 }
 
 int d2h(double* host, const double* dev, hipStream_t st) {

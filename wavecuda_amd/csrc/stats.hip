@@ -26,7 +26,8 @@ namespace wh {
 
 // ---------------------------------------------------------------- threshold
 template <typename T>
-__global__ void thresh_dwt_kernel(T* __restrict__ y, int64_t n, int J, Thr1 th) {
+__global__ void thresh_dwt_kernel(T* __restrict__ y0, int64_t n, int J, Thr1 th) {
+  T* y = y0 + int64_t(blockIdx.y) * n;
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
     if (i == 0) continue;
@@ -38,7 +39,8 @@ __global__ void thresh_dwt_kernel(T* __restrict__ y, int64_t n, int J, Thr1 th) 
 }
 
 template <typename T>
-__global__ void thresh_modwt_kernel(T* __restrict__ xmod, int64_t n, Thr1 th) {
+__global__ void thresh_modwt_kernel(T* __restrict__ xmod0, int64_t n, int J, Thr1 th) {
+  T* xmod = xmod0 + int64_t(blockIdx.y) * 2 * n * J;
   const int64_t cnt = int64_t(th.maxlevel - th.minlevel + 1) * n;
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < cnt; e += stride) {
@@ -49,26 +51,28 @@ __global__ void thresh_modwt_kernel(T* __restrict__ xmod, int64_t n, Thr1 th) {
   }
 }
 
-int threshold_inplace(void* c, int64_t n, int J, int ttype, int dtype, const wh_thresh* thr,
-                      hipStream_t st) {
+int threshold_inplace(void* c, int64_t n, int64_t count, int J, int ttype, int dtype,
+                      const wh_thresh* thr, hipStream_t st) {
   Thr1 th{thr->hardness, thr->minlevel, thr->maxlevel, thr->value, thr->levels};
   const int nt = 256;
+  if (count < 1 || count > 65535) return set_error(WH_ERR_ARG, "threshold: signal count out of range");
   const int64_t work = (ttype == kDWT) ? n : int64_t(thr->maxlevel - thr->minlevel + 1) * n;
   const unsigned blocks = unsigned(std::min<int64_t>((work + nt - 1) / nt, 4096));
+  const dim3 grid(blocks, unsigned(count));
   if (ttype == kDWT) {
     if (dtype == WH_F32)
-      hipLaunchKernelGGL(thresh_dwt_kernel<float>, dim3(blocks), dim3(nt), 0, st,
+      hipLaunchKernelGGL(thresh_dwt_kernel<float>, grid, dim3(nt), 0, st,
                          static_cast<float*>(c), n, J, th);
     else
-      hipLaunchKernelGGL(thresh_dwt_kernel<double>, dim3(blocks), dim3(nt), 0, st,
+      hipLaunchKernelGGL(thresh_dwt_kernel<double>, grid, dim3(nt), 0, st,
                          static_cast<double*>(c), n, J, th);
   } else {
     if (dtype == WH_F32)
-      hipLaunchKernelGGL(thresh_modwt_kernel<float>, dim3(blocks), dim3(nt), 0, st,
-                         static_cast<float*>(c), n, th);
+      hipLaunchKernelGGL(thresh_modwt_kernel<float>, grid, dim3(nt), 0, st,
+                         static_cast<float*>(c), n, J, th);
     else
-      hipLaunchKernelGGL(thresh_modwt_kernel<double>, dim3(blocks), dim3(nt), 0, st,
-                         static_cast<double*>(c), n, th);
+      hipLaunchKernelGGL(thresh_modwt_kernel<double>, grid, dim3(nt), 0, st,
+                         static_cast<double*>(c), n, J, th);
   }
   WH_HIP_CHECK(hipGetLastError());
   return WH_OK;

@@ -39,10 +39,19 @@ int stats_get(size_t bytes, void** out);
 void allow_dynamic_lds(const void* kern);
 
 // transform back ends (device pointers)
-int modwt_forward(const void* x, void* xmod, int64_t n, int J, int filt, int dtype,
-                  hipStream_t st);
-int modwt_inverse(const void* xmod, void* x, int64_t n, int J, int filt, int dtype,
+// MODWT of `count` signals of n samples (x, out: n apart; xmod: 2nJ apart)
+int modwt_forward(const void* x, void* xmod, int64_t n, int64_t count, int J, int filt,
+                  int dtype, hipStream_t st);
+int modwt_inverse(const void* xmod, void* x, int64_t n, int64_t count, int J, int filt, int dtype,
                   const wh_thresh* thr, hipStream_t st);
+// adjoints: of the inverse (x-cotangent -> xmod-cotangent, the fused forward
+// with exact 2^-j scalings) and of the forward (xmod-cotangent -> x-cotangent)
+int modwt_inverse_adjoint(const void* gx, void* gxmod, int64_t n, int64_t count, int J, int filt,
+                          int dtype, hipStream_t st);
+int modwt_forward_adjoint(const void* gxmod, void* gx, int64_t n, int64_t count, int J, int filt,
+                          int dtype, hipStream_t st);
+int thresh_grad(const void* c, const void* gy, void* gc, double* glam, int64_t n, int64_t count,
+                int J, int ttype, int dtype, const wh_thresh* thr, hipStream_t st);
 std::string modwt_plan_string(bool inverse, int64_t n, int J, int filt, int dtype);
 
 int dwt_forward(const void* x, void* y, int64_t n, int J, int filt, int dtype, hipStream_t st);

@@ -41,8 +41,10 @@ struct SelState {
 
 constexpr int kMaxPartials = 1024;
 
-int threshold_inplace(void* c, int64_t n, int J, int ttype, int dtype, const wh_thresh* thr,
-                      hipStream_t st);
+// in-place threshold of `count` transformed signals (DWT: n apart, MODWT:
+// 2nJ apart)
+int threshold_inplace(void* c, int64_t n, int64_t count, int J, int ttype, int dtype,
+                      const wh_thresh* thr, hipStream_t st);
 int select_median(const SelSrc& s, int dtype, bool deviation, SelState* st, double factor,
                   double* out, hipStream_t stream);
 int select_mad(const SelSrc& s, int dtype, SelState* st, double factor, double* out,
