@@ -9,10 +9,22 @@ headline value is whole-job throughput in Msamples/s.  Alongside:
                 (8 (1 + 2J) B per sample: read x once, write S_j and D_j for
                 every level) / its average duration, timed with HIP events
                 on the stream the kernel runs on;
+  batched       8192 mixed signals smoothed by one persistent launch
+                (replayed from a HIP graph) against the plain loop of one
+                smoothing call per signal, FP64 and FP32, outputs compared
+                bit for bit;
+  sweep         every filter, DWT full depth and MODWT J = 10, n = 2^20 and
+                2^24, forward and inverse (FP64);
+  small         D4 DWT forward + inverse at n = 1024 (latency);
+  smooth        single-signal universal / CV smoothing at n = 2^18;
+  differentiable  per-level thresholds fitted by Adam on 512 x 2^14 FP32
+                signals, against CV smoothing;
   cpu_baseline  the CPU oracle (the reference's serial CPU path restated)
-                timed on this host on a bounded sample of the same workload.
+                timed on this host on a bounded sample of the headline
+                workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                       [--no-batch] [--no-extra] [--plain-sample N]
 """
 from __future__ import annotations
 
@@ -168,6 +180,158 @@ def batch_section(steps, warmup, plain_sample):
     return out
 
 
+def _time_ms(fn, steps, warmup=2):
+    """Average milliseconds of fn() over `steps` runs (HIP events on torch's
+    current stream, the stream the wrappers launch on)."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def sweep_section(steps):
+    """Single-signal transforms, FP64, white noise: every filter, DWT at full
+    depth and MODWT J = 10, n = 2^20 and 2^24; forward and inverse timed
+    separately, priced at their algorithmic bytes."""
+    import torch
+
+    from wavecuda_amd import _lib as W
+    from wavecuda_amd import device as D
+
+    rows = []
+    for lg in (20, 24):
+        n = 1 << lg
+        x = torch.randn(n, dtype=torch.float64, device="cuda",
+                        generator=torch.Generator(device="cuda").manual_seed(lg))
+        for f in (W.HAAR, W.D4, W.C6, W.LA8):
+            Jd = D.resolve_levels(n, 0, f)
+            y = torch.empty_like(x)
+            xr = torch.empty_like(x)
+            tf = _time_ms(lambda: D.dwt_fwd(x, Jd, f, out=y), steps)
+            ti = _time_ms(lambda: D.dwt_inv(y, Jd, f, out=xr), steps)
+            b = 16 * n
+            rows.append({"transform": "DWT", "filter": W.FILTER_NAMES[f], "n": n, "levels": Jd,
+                         "fwd_ms": round(tf, 4), "inv_ms": round(ti, 4),
+                         "fwd_GBs": round(b / tf / 1e6, 1), "inv_GBs": round(b / ti / 1e6, 1),
+                         "msamples_per_s": round(n / (tf + ti) / 1e3, 1)})
+            del y, xr
+            J = 10
+            xm = torch.empty(2 * n * J, dtype=torch.float64, device="cuda")
+            xr = torch.empty_like(x)
+            tf = _time_ms(lambda: D.modwt_fwd(x, J, f, out=xm), steps)
+            ti = _time_ms(lambda: D.modwt_inv(xm, n, J, f, out=xr), steps)
+            bf, bi = modwt_bytes_fwd(n, J), modwt_bytes_inv(n, J)
+            rows.append({"transform": "MODWT", "filter": W.FILTER_NAMES[f], "n": n, "levels": J,
+                         "fwd_ms": round(tf, 4), "inv_ms": round(ti, 4),
+                         "fwd_GBs": round(bf / tf / 1e6, 1), "inv_GBs": round(bi / ti / 1e6, 1),
+                         "msamples_per_s": round(n / (tf + ti) / 1e3, 1)})
+            del xm, xr
+        del x
+        torch.cuda.empty_cache()
+    return rows
+
+
+def small_section(steps):
+    """D4 DWT forward + inverse, full depth (9 levels), n = 1024 doppler +
+    noise at SNR 5: a launch-latency case, reported in microseconds."""
+    import torch
+
+    from wavecuda_amd import device as D
+    from wavecuda_amd.signals import noisy_doppler
+
+    x = torch.tensor(noisy_doppler(1024, 5.0, seed=10)[0], device="cuda")
+    y = torch.empty_like(x)
+    xr = torch.empty_like(x)
+
+    def fi():
+        D.dwt_fwd(x, 0, "D4", out=y)
+        D.dwt_inv(y, 0, "D4", out=xr)
+    ms = _time_ms(fi, steps * 10, warmup=10)
+    return {"workload": "D4 DWT fwd+inv, n=1024, 9 levels, FP64", "us_per_fwd_inv": round(ms * 1e3, 2)}
+
+
+def smooth_section():
+    """Single-signal smoothing, n = 2^18 doppler + noise, MODWT J = 10, Haar
+    and LA8, soft and hard, universal and cross-validated thresholds, SNR 5
+    and 2: wall time of one synchronous call and the MSE against the clean
+    signal."""
+    import torch
+
+    from wavecuda_amd import _lib as W
+    from wavecuda_amd import device as D
+    from wavecuda_amd.signals import noisy_doppler
+
+    rows = []
+    n = 1 << 18
+    for snr in (5.0, 2.0):
+        xs, clean = noisy_doppler(n, snr, seed=10)
+        x = torch.tensor(xs, device="cuda")
+        cl = torch.tensor(clean, device="cuda")
+        for f in (W.HAAR, W.LA8):
+            for tt, tname in ((W.UNIV, "univ"), (W.CV, "cv")):
+                for hard in (W.SOFT, W.HARD):
+                    D.smooth(x, 10, W.MODWT, f, tt, hardness=hard)  # warm
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    y, t = D.smooth(x, 10, W.MODWT, f, tt, hardness=hard)
+                    wall = time.perf_counter() - t0
+                    rows.append({"snr": snr, "filter": W.FILTER_NAMES[f], "thresh": tname,
+                                 "hardness": "soft" if hard == W.SOFT else "hard",
+                                 "ms": round(wall * 1e3, 3), "threshold": t,
+                                 "mse_vs_clean": torch.mean((y - cl) ** 2).item()})
+    return rows
+
+
+def diff_section(steps=300):
+    """Per-level soft thresholds fitted by Adam: 512 doppler + noise signals
+    (SNR ~ U[1, 10]) of n = 2^14 in FP32, LA8 MODWT J = 8, loss = mean
+    squared error of the smoothed signals against the clean ones, lambda
+    initialised to the universal threshold, 300 Adam steps at lr 1e-2.
+    Compared with cross-validated smoothing (one CV threshold per signal)."""
+    import numpy as np
+    import torch
+
+    from wavecuda_amd import _lib as W
+    from wavecuda_amd import autograd as A
+    from wavecuda_amd import device as D
+    from wavecuda_amd.signals import doppler
+
+    B, n, J = 512, 1 << 14, 8
+    rng = np.random.Generator(np.random.PCG64(55))
+    clean = doppler(n)
+    sd = clean.std(ddof=1)
+    snr = rng.uniform(1.0, 10.0, size=B)
+    xs = clean[None, :] + (sd / snr)[:, None] * rng.standard_normal((B, n))
+    x = torch.tensor(xs, dtype=torch.float32, device="cuda")
+    cl = torch.tensor(np.broadcast_to(clean, (B, n)).copy(), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lam, losses = A.fit_level_thresholds(x, cl, J, W.LA8, steps=steps, lr=1e-2)
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    mse_cv = []
+    for b in range(B):
+        y, _ = D.smooth(x[b], J, W.MODWT, W.LA8, W.CV)
+        mse_cv.append(torch.mean((y - cl[b]) ** 2).item())
+    cv_s = time.perf_counter() - t0
+    return {"workload": "512 x n=2^14 FP32 doppler+noise (SNR~U[1,10]), LA8 MODWT J=8, "
+                        "per-level soft thresholds, 300 Adam steps lr 1e-2",
+            "ms_per_adam_step": round(fit_s * 1e3 / steps, 3),
+            "mse_universal_init": losses[0], "mse_fitted": losses[-1],
+            "mse_cv_gpusmooth": float(np.mean(mse_cv)),
+            "cv_smooth_ms_per_signal": round(cv_s * 1e3 / B, 3),
+            "lambda": [round(v, 5) for v in lam.cpu().tolist()]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,6 +341,8 @@ def main():
     ap.add_argument("--levels", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the sweep, small, smoothing and differentiable sections")
     ap.add_argument("--plain-sample", type=int, default=8192,
                     help="signals of the batch run through the plain per-signal loop")
     args = ap.parse_args()
@@ -284,6 +450,11 @@ def main():
     }
     if not args.no_batch:
         out["batched"] = batch_section(max(3, args.steps // 5), 2, args.plain_sample)
+    if not args.no_extra:
+        out["sweep"] = sweep_section(max(5, args.steps // 5))
+        out["small"] = small_section(max(5, args.steps // 5))
+        out["smooth"] = smooth_section()
+        out["differentiable"] = diff_section()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(J=J)
     print(json.dumps(out))

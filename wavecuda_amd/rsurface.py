@@ -1,0 +1,237 @@
+"""The reference package's R surface, restated in Python over the `.C` symbols.
+
+GPUTransform, GPUReconstruct, GPUSmooth (R/rfunctions.R:383-406, :452-468,
+:588-612) and a GPU counterpart of CPUThreshold (:508-526), with the same
+argument names (R's dots become underscores), the same checks and error
+messages (check.trans.inputs :64-119, check.thresh.inputs :168-202,
+check.smooth.inputs :211-249) and the same returned WST object (return.trans
+:259-292, return.thresh :300-308): fields x, ttype, filt, filtlen, nlevels,
+len, w, where a MODWT's w is the 2 n log2(n) vector R allocates
+(:94) with the 2 n J coefficients first.
+
+The transforms and smoothing run through `RgpuTransform` / `RgpuSmooth`
+called exactly the way R's `.C` calls them (host double vectors, pointers
+to one int each, results written in place), so this module also exercises
+the drop-in boundary.  Differences from the reference, all additions: C6
+and LA8 MODWT are accepted (the reference stops at :114-116), and
+GPUSmooth accepts "univ" and every filter (the reference stops at
+:592-594).  R's stop() is raised as ValueError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as W
+
+_TTYPE = {"DWT": 0, "MODWT": 1}
+_FILT = {"Haar": 1, "D4": 2, "C6": 3, "LA8": 4}
+_FLEN = {"Haar": 2, "D4": 4, "C6": 6, "LA8": 8}
+_SENSE = {"FWD": 1, "BWD": 0}
+_HARD = {"hard": 0, "soft": 1}
+_THRESH = {"manual": 0, "univ": 1, "cv": 2}
+
+
+@dataclass
+class WST:
+    """R's WST object (R/rfunctions.R:267-284)."""
+    x: np.ndarray | None
+    ttype: str
+    filt: str
+    filtlen: int
+    nlevels: int
+    len: int
+    w: np.ndarray = field(repr=False)
+
+
+def _is_pow2(v: float) -> bool:
+    return v > 0 and math.log2(v) == round(math.log2(v))
+
+
+def check_trans_inputs(xin, direction, nlevels, transform_type, filter):
+    """check.trans.inputs (R/rfunctions.R:64-119)."""
+    if transform_type not in _TTYPE:
+        raise ValueError("Unexpected transform type")
+    ttype = _TTYPE[transform_type]
+    if filter not in _FILT:
+        raise ValueError("Unexpected filter")
+    filt, filtlen = _FILT[filter], _FLEN[filter]
+    if direction not in _SENSE:
+        raise ValueError("Unexpected direction!")
+    sense = _SENSE[direction]
+    xin = np.asarray(xin)
+    if not np.issubdtype(xin.dtype, np.number):
+        raise ValueError("Unexpected input vector")
+    lenxin = xin.size
+    if ttype == 0:
+        x = np.array(xin, dtype=np.float64)
+        xmod = np.zeros(0)
+        length = lenxin
+        if not _is_pow2(length):
+            raise ValueError("Need length to be power of 2")
+    elif sense == 1:
+        x = np.array(xin, dtype=np.float64)
+        length = lenxin
+        if not _is_pow2(length):
+            raise ValueError("Need length to be power of 2")
+        xmod = np.zeros(int(length * 2 * math.log2(length)))
+    else:
+        xmod = np.array(xin, dtype=np.float64)
+        if nlevels == 0:
+            raise ValueError("Probably got an error of infinite length: len = Inf")
+        length = lenxin / (2 * nlevels)
+        if not _is_pow2(length):
+            raise ValueError("Was this really a MODWT vector? If so, check nlevels. I calculate "
+                             "the reconstruction vector to be of inappropriate size.")
+        length = int(length)
+        x = np.zeros(length)
+    maxlevels = math.floor(math.log2(length / filtlen)) + 1
+    if nlevels == 0:
+        nlevels = maxlevels
+    if nlevels != round(nlevels) or nlevels < 0 or nlevels > maxlevels:
+        raise ValueError(f"nlevels should be an integer [whole number] between 1 and {maxlevels}")
+    return {"x": x, "xmod": xmod, "len": int(length), "sense": sense, "nlevels": int(nlevels),
+            "ttype": ttype, "filt": filt, "filtlen": filtlen}
+
+
+def prepare_wst_arg_list(xwav: WST):
+    """prepare.wst.arg.list (R/rfunctions.R:128-155)."""
+    base = {"len": xwav.len, "sense": 0, "nlevels": xwav.nlevels,
+            "ttype": _TTYPE[xwav.ttype], "filt": _FILT[xwav.filt], "filtlen": xwav.filtlen}
+    if xwav.ttype == "DWT":
+        return {"x": np.array(xwav.w, dtype=np.float64), "xmod": np.zeros(0), **base}
+    x = np.array(xwav.x if xwav.x is not None else np.zeros(xwav.len), dtype=np.float64)
+    return {"x": x, "xmod": np.array(xwav.w, dtype=np.float64), **base}
+
+
+def check_thresh_inputs(xwav: WST, hard_soft, thresh, min_level, max_level):
+    """check.thresh.inputs (R/rfunctions.R:168-202)."""
+    a = prepare_wst_arg_list(xwav)
+    if np.ndim(thresh) != 0:
+        raise ValueError("Only scalar thresholds supported")
+    if thresh <= 0:
+        raise ValueError("Threshold must be greater than 0")
+    a["thresh"] = float(thresh)
+    del a["sense"]
+    if hard_soft not in _HARD:
+        raise ValueError("Only hard or soft thresholding supported")
+    a["hardness"] = _HARD[hard_soft]
+    if (min_level != round(min_level) or max_level != round(max_level) or min_level > max_level
+            or min_level <= 0 or max_level > xwav.nlevels):
+        raise ValueError("min.level and max.level must be integers in the range 1...nlevels")
+    a["min_level"] = int(min_level) - 1
+    a["max_level"] = int(max_level) - 1
+    return a
+
+
+def check_smooth_inputs(xin, nlevels, transform_type, filter, thresh_type, thresh, hard_soft,
+                        min_level, max_level, tol):
+    """check.smooth.inputs (R/rfunctions.R:211-249)."""
+    a = check_trans_inputs(xin, "FWD", nlevels, transform_type, filter)
+    del a["sense"]
+    if thresh_type not in _THRESH:
+        raise ValueError("Only manual, univ or cv supported for threshold type")
+    a["threshtype"] = _THRESH[thresh_type]
+    if thresh_type == "manual":
+        if thresh is None or np.ndim(thresh) != 0:
+            raise ValueError("Only scalar thresholds supported")
+        if thresh <= 0:
+            raise ValueError("Threshold must be greater than 0")
+        a["thresh"] = float(thresh)
+    else:
+        a["thresh"] = None
+    if thresh_type == "cv":
+        if tol <= 0:
+            raise ValueError("Tolerance must be greater than 0")
+        a["tol"] = float(tol)
+    else:
+        a["tol"] = None
+    if hard_soft not in _HARD:
+        raise ValueError("Only hard or soft thresholding supported")
+    a["hardness"] = _HARD[hard_soft]
+    if (min_level != round(min_level) or max_level != round(max_level) or min_level > max_level
+            or min_level <= 0 or max_level > nlevels):
+        raise ValueError("min.level and max.level must be integers in the range 1...nlevels")
+    a["min_level"] = int(min_level) - 1
+    a["max_level"] = int(max_level) - 1
+    return a
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(v):
+    return C.byref(C.c_int(int(v)))
+
+
+def _call_transform(a):
+    """.C("RgpuTransform", x, xmod, len, sense, nlevels, ttype, filter,
+    filterlen) -- R/rfunctions.R:394-403."""
+    x = np.ascontiguousarray(a["x"], dtype=np.float64)
+    xmod = np.ascontiguousarray(a["xmod"], dtype=np.float64)
+    W.lib().RgpuTransform(_dp(x), _dp(xmod) if xmod.size else None, _ip(a["len"]),
+                          _ip(a["sense"]), _ip(a["nlevels"]), _ip(a["ttype"]), _ip(a["filt"]),
+                          _ip(a["filtlen"]))
+    out = dict(a)
+    out["x"], out["xmod"] = x, xmod
+    return out
+
+
+def return_trans(arglist, argsin):
+    """return.trans (R/rfunctions.R:259-292)."""
+    if arglist["sense"] == 1:
+        w = arglist["x"] if argsin["ttype"] == "DWT" else arglist["xmod"]
+        return WST(x=argsin.get("x"), ttype=argsin["ttype"], filt=argsin["filter"],
+                   filtlen=arglist["filtlen"], nlevels=arglist["nlevels"], len=arglist["len"],
+                   w=w)
+    return arglist["x"]
+
+
+def GPUTransform(xin, direction="FWD", nlevels=0, transform_type="DWT", filter="Haar"):
+    """GPUTransform (R/rfunctions.R:383-406): a WST for FWD, a vector for BWD."""
+    args_in = {"ttype": transform_type, "filter": filter}
+    if direction == "FWD":
+        args_in["x"] = np.asarray(xin)
+    a = check_trans_inputs(xin, direction, nlevels, transform_type, filter)
+    return return_trans(_call_transform(a), args_in)
+
+
+def GPUReconstruct(xwav: WST):
+    """GPUReconstruct (R/rfunctions.R:452-468)."""
+    return _call_transform(prepare_wst_arg_list(xwav))["x"]
+
+
+def GPUThreshold(xwav: WST, thresh, hard_soft, min_level, max_level):
+    """The GPU counterpart of CPUThreshold (R/rfunctions.R:508-526): the
+    WST with its coefficients thresholded on the device."""
+    import torch
+
+    from . import device as D
+
+    a = check_thresh_inputs(xwav, hard_soft, thresh, min_level, max_level)
+    w = torch.tensor(np.asarray(xwav.w, dtype=np.float64), device="cuda")
+    D.threshold_(w, xwav.len, xwav.nlevels, a["ttype"], a["filt"], a["thresh"], a["hardness"],
+                 a["min_level"], a["max_level"])
+    out = WST(**{**xwav.__dict__})
+    out.w = w.cpu().numpy()
+    return out
+
+
+def GPUSmooth(xin, nlevels, transform_type, filter, thresh_type, thresh=None, hard_soft="soft",
+              min_level=1, max_level=None, tol=0.01):
+    """GPUSmooth (R/rfunctions.R:588-612): the smoothed vector."""
+    if max_level is None:
+        max_level = nlevels
+    a = check_smooth_inputs(xin, nlevels, transform_type, filter, thresh_type, thresh, hard_soft,
+                            min_level, max_level, tol)
+    x = np.ascontiguousarray(a["x"], dtype=np.float64)
+    t = np.array([a["thresh"] if a["thresh"] is not None else 0.0])
+    tl = np.array([a["tol"] if a["tol"] is not None else 0.01])
+    W.lib().RgpuSmooth(_dp(x), _ip(a["len"]), _ip(a["nlevels"]), _ip(a["ttype"]), _ip(a["filt"]),
+                       _ip(a["filtlen"]), _ip(a["threshtype"]), _dp(t), _ip(a["hardness"]),
+                       _ip(a["min_level"]), _ip(a["max_level"]), _dp(tl))
+    return x
