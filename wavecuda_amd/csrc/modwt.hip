@@ -98,7 +98,7 @@ int launch(bool inverse, int filt, const Group& G, const ModwtArgs& a, int64_t c
   // (16-byte LDS and HBM accesses; it is write-bound), two outputs of one
   // residue class for the inverse (L + 1 interleaved (S, D) reads instead of
   // 2L; it is LDS-bound)
-  gl.map = inverse ? 1 : 0;
+  gl.map = inverse ? 2 : 0;
   if (gl.blocks > 0x7fffffff) return set_error(WH_ERR_ARG, "modwt: grid too large");
   switch (filt) {
     case kHAAR: return modwt_launch<T, kHAAR>(inverse, gl, a, st);

@@ -617,19 +617,176 @@ __device__ __forceinline__ void modwt_inv_body(const ModwtArgs& a, int64_t rb, i
   }
 }
 
+// ------------------------------------------------- inverse, linear buffers
+// The R = 1 group with the residue-class mapping, but each level keeps a
+// linear buffer instead of a ring: slots [0, W_l) hold the window (the
+// W_l positions before the chunk) and [W_l, W_l + M) the chunk, so
+// position c + q sits at slot W_l + q and every tap of a thread is a
+// compile-time offset from one per-level base address -- no wrap
+// arithmetic per tap.  At the end of a chunk the last W_l slots of the
+// chunk region move to [0, W_l) (through registers, since the regions
+// overlap when W_l > M) before the next chunk is parked behind them.
+// Slots interleave (S, D) as in the ring version.
+template <typename T, int FILT, int G, int NT>
+struct InvLinCtx {
+  using V = typename Vec2<T>::type;
+  static constexpr int L = Filter<FILT>::L;
+  T* lds;
+  const ModwtArgs* a;
+  const T* xmod;
+  const T* stop;
+  T* out;
+  int M, mm;
+  int64_t n, c;
+  V pre[G + 1];  // next chunk: D_l (l = 1..G) and S_G (index 0)
+
+  __device__ __forceinline__ int off(int l) const { return inv_off(L, M, l); }
+  __device__ __forceinline__ T thr(int l) const {
+    const int lev = l - 1;
+    if (!a->th_on || lev < a->minlevel || lev > a->maxlevel) return T(-1);
+    return a->th_levels ? T(a->th_levels[lev]) : T(a->th_value);
+  }
+  __device__ __forceinline__ V thr2(V d, T t) const {
+    if (t >= T(0)) {
+      d.x = thresh_coef(d.x, t, a->hardness);
+      d.y = thresh_coef(d.y, t, a->hardness);
+    }
+    return d;
+  }
+  __device__ __forceinline__ void load_next(int64_t cc) {
+    const int64_t i = (cc + mm) & (n - 1);
+#pragma unroll
+    for (int l = 1; l <= G; ++l) pre[l] = ldv(xmod + 2 * n * int64_t(l - 1) + n + i);
+    pre[0] = ldv(stop + i);
+  }
+  // the loaded chunk goes to slots W_l + mm, mm + 1 of every level
+  __device__ __forceinline__ void park() {
+#pragma unroll
+    for (int l = 1; l <= G; ++l) {
+      T* p = lds + 2 * (off(l) + inv_win(L, l) + mm);
+      const V d = thr2(pre[l], thr(l));
+      if (l == G) {
+        stv(p, V{pre[0].x, d.x});
+        stv(p + 2, V{pre[0].y, d.y});
+      } else {
+        p[1] = d.x;
+        p[3] = d.y;
+      }
+    }
+  }
+  template <int l, bool STORE>
+  __device__ __forceinline__ void level() {
+    constexpr int s = 1 << (l - 1);
+    constexpr int W = inv_win(L, l);
+    const int t = mm >> 1;
+    const int q0 = (s == 1) ? mm : 2 * s * (t / s) + (t % s);
+    // taps of outputs q0 and q0 + s: positions q0 + u s, u = -(L-1)..1
+    const T* b = lds + 2 * (off(l) + W + q0 - (L - 1) * s);
+    T ws[L + 1], wd[L + 1];
+#pragma unroll
+    for (int u = 0; u <= L; ++u) {
+      const V v = ldv(b + 2 * u * s);
+      ws[u] = v.x;
+      wd[u] = v.y;
+    }
+    const T e0 = synthesis<T, FILT>(ws, wd, L - 1);
+    const T e1 = synthesis<T, FILT>(ws, wd, L);
+    if constexpr (l > 1) {
+      T* o = lds + 2 * (off(l - 1) + inv_win(L, l - 1) + q0);
+      o[0] = e0;
+      o[2 * s] = e1;
+    } else if constexpr (STORE) {
+      stv_nt(out + c + q0, V{e0, e1});
+    }
+    __syncthreads();
+  }
+  template <int l, bool STORE>
+  __device__ __forceinline__ void levels() {
+    if constexpr (l >= 1) {
+      level<l, STORE>();
+      levels<l - 1, STORE>();
+    }
+  }
+  // move slots [M, M + W_l) to [0, W_l) for every level, then park
+  template <int l>
+  __device__ __forceinline__ void shift_and_park() {
+    if constexpr (l > G) {
+      __syncthreads();  // every old slot read before any is overwritten
+      park();
+    } else {
+      constexpr int W = inv_win(L, l);
+      constexpr int K = (W + NT - 1) / NT;
+      V tmp[K];
+      const int o = off(l);
+      const int tid = threadIdx.x;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int i = tid + k * NT;
+        if (i < W) tmp[k] = ldv(lds + 2 * (o + M + i));
+      }
+      shift_and_park<l + 1>();
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int i = tid + k * NT;
+        if (i < W) stv(lds + 2 * (o + i), tmp[k]);
+      }
+    }
+  }
+};
+
+template <typename T, int FILT, int NT, int G>
+__device__ __forceinline__ void modwt_inv_lin_body(const ModwtArgs& a, int64_t seg,
+                                                   unsigned char* smem_raw) {
+  InvLinCtx<T, FILT, G, NT> C;
+  C.lds = reinterpret_cast<T*>(smem_raw);
+  C.a = &a;
+  C.xmod = static_cast<const T*>(a.xmod);
+  C.stop = static_cast<const T*>(a.stop);
+  C.out = static_cast<T*>(a.out);
+  C.M = a.M;
+  C.n = a.n;
+  C.mm = (2 * int(threadIdx.x)) % a.M;
+  const int64_t cbot = seg * a.P - a.warm;
+
+  C.load_next(cbot);
+  C.park();
+  __syncthreads();
+  const int nwarm = a.warm / a.M;
+  int k = 0;
+  for (; k < nwarm; ++k) {
+    C.c = cbot + int64_t(k) * a.M;
+    C.load_next(C.c + a.M);
+    C.template levels<G, false>();
+    C.template shift_and_park<1>();
+    __syncthreads();
+  }
+  for (; k < a.nchunks; ++k) {
+    C.c = cbot + int64_t(k) * a.M;
+    C.load_next(C.c + a.M);  // wraps harmlessly on the last chunk
+    C.template levels<G, true>();
+    C.template shift_and_park<1>();
+    __syncthreads();
+  }
+}
+
 template <typename T, int FILT, int R, int NT, int G, int MAP>
 __global__ __launch_bounds__(NT) void modwt_inv_k(ModwtArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int64_t nrb = (int64_t(1) << a.j0) / R;
   const ModwtArgs b = batch_slice<T>(a);
-  modwt_inv_body<T, FILT, R, NT, G, MAP>(b, int64_t(blockIdx.x) % nrb, int64_t(blockIdx.x) / nrb,
-                                         smem_raw);
+  if constexpr (MAP == 2) {
+    static_assert(R == 1, "linear buffers serve the first group");
+    modwt_inv_lin_body<T, FILT, NT, G>(b, int64_t(blockIdx.x), smem_raw);
+  } else {
+    modwt_inv_body<T, FILT, R, NT, G, MAP>(b, int64_t(blockIdx.x) % nrb,
+                                           int64_t(blockIdx.x) / nrb, smem_raw);
+  }
 }
 
 // Launch of a (T, FILT) group kernel; defined by instantiation files.
 struct GroupLaunch {
   int R, NT, G;
-  int map;  // R = 1 element mapping: 0 adjacent pairs, 1 two outputs of one residue class (set by the planner)
+  int map;  // R = 1 element mapping: 0 adjacent pairs, 1 two outputs of one residue class (set by the planner), 2 the same with linear buffers (inverse)
   int sp;   // forward store policy (kStoreAll or kStoreAdjoint)
   int64_t blocks;
   int64_t count;  // signals (grid y)

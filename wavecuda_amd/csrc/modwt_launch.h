@@ -19,7 +19,7 @@ template <typename T, int FILT, int R, int NT, int G>
 int modwt_launch_one(bool inverse, const GroupLaunch& gl, const ModwtArgs& a, hipStream_t st) {
   const dim3 grid(unsigned(gl.blocks), unsigned(gl.count));
   if (inverse) {
-    constexpr int MAP = (R == 1) ? 1 : 0;
+    constexpr int MAP = (R == 1) ? 2 : 0;
     auto kern = modwt_inv_k<T, FILT, R, NT, G, MAP>;
     modwt_allow_lds(kern);
     hipLaunchKernelGGL(kern, grid, dim3(NT), gl.lds, st, a);
