@@ -15,7 +15,7 @@
 //   MODWT: the fused sliding-window forward (modwt_fwd_body) writes the J
 //   levels into scratch, block_mad takes the exact MAD of D_1 (the
 //   reference's univ_thresh(w, 0, 0): src/thresh.cpp:170-201,
-//   src/rfunctions.cu:82-87), and the fused inverse (modwt_inv_body)
+//   src/rfunctions.cu:82-87), and the fused inverse (modwt_inv_lin_body)
 //   thresholds every detail level as it reads it.
 //   DWT: levels run on compact sequences, through scratch while they are
 //   longer than the LDS tile and in LDS afterwards; the MAD is taken over
@@ -101,7 +101,7 @@ __device__ __forceinline__ void batch_modwt_item(const BatchArgs& b, const Batch
   a.maxlevel = G - 1;
   a.th_value = t;
   a.th_levels = nullptr;
-  modwt_inv_body<T, FILT, 1, NT, G, 1>(a, 0, 0, smem);
+  modwt_inv_lin_body<T, FILT, NT, G>(a, 0, smem);
   batch_phase();
 }
 
@@ -120,6 +120,7 @@ __device__ __forceinline__ void batch_dwt_item(const BatchArgs& b, const BatchIt
                                                unsigned char* smem) {
   constexpr int L = Filter<FILT>::L;
   constexpr int kBatchLds = batch_lds_elems<T>();
+  constexpr int kU = 4;  // global-level pairs per thread in flight
   T* lA = reinterpret_cast<T*>(smem);
   T* lB = lA + kBatchLds;
   const int tid = threadIdx.x;
@@ -138,14 +139,25 @@ __device__ __forceinline__ void batch_dwt_item(const BatchArgs& b, const BatchIt
     bool toA = true;
     while (j <= J && N > kBatchLds) {
       T* dst = toA ? gA : gB;
-      for (int64_t m = tid; m < N / 2; m += NT) {
-        T v[L];
+      // kU pairs per thread in flight: loads first, then the arithmetic
+      for (int64_t m0 = tid; m0 < N / 2; m0 += int64_t(NT) * kU) {
+        T v[kU][L];
 #pragma unroll
-        for (int k = 0; k < L; ++k) v[k] = src[(2 * m + k) & (N - 1)];
-        T s, d;
-        fwd_pair<T, FILT>(v, s, d);
-        dst[m] = s;
-        C[(2 * m + 1) << (j - 1)] = d;
+        for (int u = 0; u < kU; ++u) {
+          const int64_t m = m0 + int64_t(u) * NT;
+#pragma unroll
+          for (int k = 0; k < L; ++k) v[u][k] = (m < N / 2) ? src[(2 * m + k) & (N - 1)] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const int64_t m = m0 + int64_t(u) * NT;
+          if (m < N / 2) {
+            T s, d;
+            fwd_pair<T, FILT>(v[u], s, d);
+            dst[m] = s;
+            C[(2 * m + 1) << (j - 1)] = d;
+          }
+        }
       }
       batch_phase();
       src = dst;
@@ -236,18 +248,30 @@ __device__ __forceinline__ void batch_dwt_item(const BatchArgs& b, const BatchIt
       bool inA = true;
       for (; j >= 1; --j) {
         T* out = (j == 1) ? y : (inA ? gB : gA);
-        for (int64_t m = tid; m < N; m += NT) {
-          T sv[L / 2], dv[L / 2];
+        for (int64_t m0 = tid; m0 < N; m0 += int64_t(NT) * kU) {
+          T sv[kU][L / 2], dv[kU][L / 2];
 #pragma unroll
-          for (int q = 0; q < L / 2; ++q) {
-            const int64_t mm = (m - q) & (N - 1);
-            sv[q] = s[mm];
-            dv[q] = thresh_coef(C[(2 * mm + 1) << (j - 1)], tt, b.hardness);
+          for (int u = 0; u < kU; ++u) {
+            const int64_t m = m0 + int64_t(u) * NT;
+#pragma unroll
+            for (int q = 0; q < L / 2; ++q) {
+              const int64_t mm = (m - q) & (N - 1);
+              sv[u][q] = (m < N) ? s[mm] : T(0);
+              dv[u][q] = (m < N) ? C[(2 * mm + 1) << (j - 1)] : T(0);
+            }
           }
-          T e, o;
-          inv_pair<T, FILT>(sv, dv, e, o);
-          out[2 * m] = e;
-          out[2 * m + 1] = o;
+#pragma unroll
+          for (int u = 0; u < kU; ++u) {
+            const int64_t m = m0 + int64_t(u) * NT;
+            if (m < N) {
+#pragma unroll
+              for (int q = 0; q < L / 2; ++q) dv[u][q] = thresh_coef(dv[u][q], tt, b.hardness);
+              T e, o;
+              inv_pair<T, FILT>(sv[u], dv[u], e, o);
+              out[2 * m] = e;
+              out[2 * m + 1] = o;
+            }
+          }
         }
         batch_phase();
         s = out;

@@ -5,6 +5,13 @@
 // digits and an LDS histogram, tracking the two middle ranks (count-1)/2
 // and count/2 at once; median = 0.5 (a + b) and MAD = 1.4826 median|v - med|
 // as in reference src/utils.cpp:166-203.
+//
+// The values live in global memory (the workgroup's scratch).  Each pass
+// over them issues kSelUnroll independent loads per thread before using
+// any, so a pass costs a few memory latencies rather than one per element.
+// As soon as the histogram shows that the values still matching the
+// selected prefixes fit in LDS (kSelCap keys), one more pass copies their
+// keys there and the remaining digits are resolved in LDS.
 #pragma once
 
 #include "wh_common.h"
@@ -13,6 +20,8 @@ namespace wh {
 
 constexpr int kSelBits = 11;
 constexpr int kSelBins = 1 << kSelBits;
+constexpr int kSelCap = 4096;   // keys kept in LDS
+constexpr int kSelUnroll = 8;   // loads in flight per thread and pass
 
 __device__ __forceinline__ uint64_t sel_key(double v) {
   uint64_t u = __double_as_longlong(v);
@@ -32,10 +41,35 @@ __device__ __forceinline__ double sel_val(uint64_t k, float*) {
   return double(__uint_as_float(u));
 }
 
-// LDS workspace of block_median (bytes)
+// LDS layout of block_median: the candidate keys (8-byte aligned) first,
+// then the two histograms, the scan array and the selection state.
 template <int NT>
 constexpr size_t block_sel_lds() {
-  return size_t(2) * kSelBins * sizeof(unsigned) + size_t(NT) * sizeof(unsigned) + 64;
+  return size_t(kSelCap) * 8 + size_t(2) * kSelBins * 4 + size_t(NT) * 4 + 128;
+}
+
+// f(key) for every value v[off + e * stride], e < count (|v - c| when
+// deviation), kSelUnroll loads in flight per thread
+template <typename T, int NT, typename F>
+__device__ __forceinline__ void sel_for_each(const T* __restrict__ v, int64_t off, int64_t stride,
+                                             int64_t count, bool deviation, T c, F f) {
+  for (int64_t b = threadIdx.x; b < count; b += int64_t(NT) * kSelUnroll) {
+    T x[kSelUnroll];
+#pragma unroll
+    for (int u = 0; u < kSelUnroll; ++u) {
+      const int64_t e = b + int64_t(u) * NT;
+      x[u] = (e < count) ? v[off + e * stride] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kSelUnroll; ++u) {
+      const int64_t e = b + int64_t(u) * NT;
+      if (e < count) {
+        T y = x[u];
+        if (deviation) y = fabs(y - c);
+        f(sel_key(y));
+      }
+    }
+  }
 }
 
 // Median of v[off + e * stride], e < count (|v - center| when deviation),
@@ -44,18 +78,24 @@ constexpr size_t block_sel_lds() {
 template <typename T, int NT>
 __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t stride, int64_t count,
                                bool deviation, double center, unsigned char* ws) {
-  unsigned* hist = reinterpret_cast<unsigned*>(ws);            // [2][kSelBins]
-  unsigned* part = hist + 2 * kSelBins;                         // [NT]
-  uint64_t* st = reinterpret_cast<uint64_t*>(part + NT);        // prefix0, prefix1, found0, found1
-  long long* ks = reinterpret_cast<long long*>(st + 4);         // k0, k1
+  uint64_t* cand = reinterpret_cast<uint64_t*>(ws);                     // [kSelCap]
+  unsigned* hist = reinterpret_cast<unsigned*>(cand + kSelCap);        // [2][kSelBins]
+  unsigned* part = hist + 2 * kSelBins;                                 // [NT]
+  uint64_t* st = reinterpret_cast<uint64_t*>(part + NT);  // prefix0, prefix1, digit0, digit1
+  long long* ks = reinterpret_cast<long long*>(st + 4);   // k0, k1
+  long long* cs = ks + 2;                                  // keys under the new prefixes
+  int* ncand = reinterpret_cast<int*>(cs + 2);
   const int tid = threadIdx.x;
   const T c = T(center);
   if (tid == 0) {
     st[0] = st[1] = 0;
     ks[0] = (count - 1) / 2;
     ks[1] = count / 2;
+    *ncand = 0;
   }
   constexpr int kbits = int(sizeof(T)) * 8;
+  bool in_lds = false;
+  int nc = 0;
   for (int hi = kbits; hi > 0;) {
     const int lo = hi > kSelBits ? hi - kSelBits : 0;
     for (int i = tid; i < 2 * kSelBins; i += NT) hist[i] = 0;
@@ -63,14 +103,16 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
     const uint64_t p0 = st[0], p1 = st[1];
     const bool same = (p0 == p1);
     const uint64_t dmask = (uint64_t(1) << (hi - lo)) - 1;
-    for (int64_t e = tid; e < count; e += NT) {
-      T x = v[off + e * stride];
-      if (deviation) x = fabs(x - c);
-      const uint64_t k = sel_key(x);
+    auto count_key = [&](uint64_t k) {
       const uint64_t top = (hi >= 64) ? 0 : (k >> hi);
       const unsigned d = unsigned((k >> lo) & dmask);
       if (top == p0) atomicAdd(&hist[d], 1u);
       if (!same && top == p1) atomicAdd(&hist[kSelBins + d], 1u);
+    };
+    if (in_lds) {
+      for (int i = tid; i < nc; i += NT) count_key(cand[i]);
+    } else {
+      sel_for_each<T, NT>(v, off, stride, count, deviation, c, count_key);
     }
     __syncthreads();
     constexpr int per = kSelBins / NT;
@@ -96,6 +138,7 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
           if (acc + cnt > k) {
             st[2 + r] = uint64_t(tid * per + i);
             ks[r] = k - acc;
+            cs[r] = cnt;
             break;
           }
           acc += cnt;
@@ -106,6 +149,20 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
       __syncthreads();
     }
     hi = lo;
+    if (!in_lds && hi > 0) {
+      const uint64_t q0 = st[0], q1 = st[1];
+      const long long total = cs[0] + (q0 != q1 ? cs[1] : 0);
+      if (total <= kSelCap) {
+        // one more pass keeps the keys under the selected prefixes in LDS
+        sel_for_each<T, NT>(v, off, stride, count, deviation, c, [&](uint64_t k) {
+          const uint64_t top = k >> hi;
+          if (top == q0 || top == q1) cand[atomicAdd(ncand, 1)] = k;
+        });
+        __syncthreads();
+        nc = *ncand;
+        in_lds = true;
+      }
+    }
   }
   const double a = sel_val(st[0], (T*)nullptr), b = sel_val(st[1], (T*)nullptr);
   __syncthreads();
