@@ -332,6 +332,20 @@ def diff_section(steps=300):
             "lambda": [round(v, 5) for v in lam.cpu().tolist()]}
 
 
+def copy_calibration(nbytes=1 << 30, reps=4):
+    """A plain device-to-device copy of nbytes, for scaling the rocprofv3
+    FETCH_SIZE / WRITE_SIZE counters: the copy moves exactly nbytes in and
+    nbytes out."""
+    import torch
+
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    ms = _time_ms(lambda: b.copy_(a), reps, warmup=1)
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps, "ms": round(ms, 4),
+                      "GBs_read_plus_write": round(2 * nbytes / ms / 1e6, 1)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -341,6 +355,8 @@ def main():
     ap.add_argument("--levels", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device copy (counter calibration)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the sweep, small, smoothing and differentiable sections")
     ap.add_argument("--plain-sample", type=int, default=8192,
@@ -362,6 +378,9 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     W.check(W.lib().wh_init(0), "wh_init")
+    if args.calibrate_copy:
+        copy_calibration()
+        return
     n = 1 << args.log2n
     J = args.levels
     filt = W.LA8
@@ -428,7 +447,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "modwt_fwd_k<double, LA8, R=1, NT=512, G=10>",
+            "kernel": "modwt_fwd_k<double, 4, 1, 512, 10, 0, 0> (LA8, R=1, 512 threads, G=10)",
             "achieved": round(fwd_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -438,7 +457,7 @@ def main():
             "avg_launch_ms": round(fwd_ms, 4),
         },
         "inverse_kernel": {
-            "kernel": "modwt_inv_k<double, LA8> (G=8 R=1, then G=2 R=32)",
+            "kernel": "modwt_inv_k<double, 4, 32, 256, 2, 0> then modwt_inv_k<double, 4, 1, 512, 8, 2>",
             "achieved_GBs": round(inv_gbs, 1),
             "frac": round(inv_gbs / HBM_PEAK_GBS, 4),
             "algorithmic_bytes_per_launch": bi,
