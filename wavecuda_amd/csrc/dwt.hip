@@ -70,19 +70,21 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
   constexpr int HMAX = halo_fwd(kGmax, L);
   constexpr int SZA = kTile + HMAX;
   constexpr int SZB = kTile / 2 + halo_fwd(kGmax - 1, L);
-  __shared__ T bufA[SZA];
-  __shared__ T bufB[SZB];
-  __shared__ T dbuf[kTile];
+  using V = typename Vec2<T>::type;
+  __shared__ __align__(16) T bufA[SZA + 2];
+  __shared__ __align__(16) T bufB[SZB + 2];
+  __shared__ __align__(16) T dbuf[kTile];
 
   const int tid = threadIdx.x;
   const int g = a.g;
   const int64_t N = a.N;
-  const int64_t base = int64_t(blockIdx.x) * kTile;
   const T* __restrict__ src = static_cast<const T*>(a.src);
   T* __restrict__ y = static_cast<T*>(a.y);
 
+  const int64_t base = int64_t(blockIdx.x) * kTile;
   const int K0 = kTile + halo_fwd(g, L);
-  for (int q = tid; q < K0; q += kNT) bufA[q] = src[(base + q) & (N - 1)];
+  for (int q = 2 * tid; q < K0; q += 2 * kNT)
+    *reinterpret_cast<V*>(bufA + q) = *reinterpret_cast<const V*>(src + ((base + q) & (N - 1)));
   __syncthreads();
 
   T* cur = bufA;
@@ -92,9 +94,14 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
     const int Kl = own + halo_fwd(g - l, L);
     const int doff = kTile - (kTile >> (l - 1));
     for (int i = tid; i < Kl; i += kNT) {
+      // taps 2i .. 2i+L-1 as L/2 aligned 16-byte pairs (no bank conflicts)
       T v[L];
 #pragma unroll
-      for (int k = 0; k < L; ++k) v[k] = cur[2 * i + k];
+      for (int k = 0; k < L / 2; ++k) {
+        const V p = *reinterpret_cast<const V*>(cur + 2 * i + 2 * k);
+        v[2 * k] = p.x;
+        v[2 * k + 1] = p.y;
+      }
       T s, d;
       fwd_pair<T, FILT>(v, s, d);
       nxt[i] = s;
@@ -107,16 +114,27 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
   }
   // cur holds s_{j0+g}; own part is cur[0 .. kTile >> g)
   const int j0 = a.j0;
-  for (int q = tid; q < kTile; q += kNT) {
-    int z = (q == 0) ? 31 : __builtin_ctz(unsigned(q));
-    T v;
-    if (z >= g) {
-      if (!(a.final_pass || a.write_coarse)) continue;
-      v = cur[q >> g];
-    } else {
-      v = dbuf[(kTile - (kTile >> z)) + (q >> (z + 1))];
+  if (j0 == 0 && a.write_coarse) {
+    // first pass: every position of the tile, as 16-byte pairs (q even:
+    // a coarser coefficient; q + 1: a level-1 detail)
+    for (int qq = tid; qq < kTile / 2; qq += kNT) {
+      const int q = 2 * qq;
+      const int z = (q == 0) ? 31 : __builtin_ctz(unsigned(q));
+      const T v0 = (z >= g) ? cur[q >> g] : dbuf[(kTile - (kTile >> z)) + (q >> (z + 1))];
+      *reinterpret_cast<V*>(y + base + q) = V{v0, dbuf[qq]};
     }
-    y[(base + q) << j0] = v;
+  } else {
+    for (int q = tid; q < kTile; q += kNT) {
+      int z = (q == 0) ? 31 : __builtin_ctz(unsigned(q));
+      T v;
+      if (z >= g) {
+        if (!(a.final_pass || a.write_coarse)) continue;
+        v = cur[q >> g];
+      } else {
+        v = dbuf[(kTile - (kTile >> z)) + (q >> (z + 1))];
+      }
+      y[(base + q) << j0] = v;
+    }
   }
   if (!a.final_pass) {
     T* __restrict__ dst = static_cast<T*>(a.dst);
@@ -131,9 +149,15 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
   constexpr int L = Filter<FILT>::L;
   constexpr int HP = L / 2 - 1;
   constexpr int PADS = 8;
-  __shared__ T bufX[kTile + PADS];
-  __shared__ T bufY[kTile / 2 + PADS];
+  // the y span the first pass needs: the tile and (L-2) 2^kGmax positions
+  // below it (the lowest detail any of its levels reads)
+  constexpr int HY = (L - 2) << kGmax;
+  using V = typename Vec2<T>::type;
+  __shared__ __align__(16) T pool[(kTile + PADS) + (kTile / 2 + PADS)];
   __shared__ T dbuf[kTile + PADS * kGmax];
+  static_assert(kTile + HY <= (kTile + PADS) + (kTile / 2 + PADS), "y staging fits the pool");
+  T* bufX = pool;
+  T* bufY = pool + kTile + PADS;
 
   const int tid = threadIdx.x;
   const int g = a.g, j0 = a.j0;
@@ -152,6 +176,16 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
     return ee;
   };
   // details d_l over local [0, own_l + e_l), global m = B_l - e_l + i
+  // The first pass (j0 = 0) reads its y span once, with contiguous 16-byte
+  // loads, into the pool and takes the strided details from there; deeper
+  // passes read their (few) details from y directly.
+  const bool staged = (j0 == 0) && (base >= HY) && (N >= kTile + HY);
+  if (staged) {
+    const T* ys = y + (base - HY);
+    for (int i = tid; i < (kTile + HY) / 2; i += kNT)
+      *reinterpret_cast<V*>(pool + 2 * i) = *reinterpret_cast<const V*>(ys + 2 * i);
+    __syncthreads();
+  }
   int doff = 0;
   for (int l = 1; l <= g; ++l) {
     const int own = kTile >> l;
@@ -163,12 +197,14 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
     const T t = thr_level<T>(a.thr, jg - 1);
     for (int i = tid; i < cnt; i += kNT) {
       int64_t m = (Bl - el + i) & (Nl - 1);
-      T d = y[(2 * m + 1) << (jg - 1)];
+      const int64_t pos = (2 * m + 1) << (jg - 1);
+      T d = staged ? pool[pos - (base - HY)] : y[pos];
       if (t >= T(0)) d = thresh_coef(d, t, a.thr.hardness);
       dbuf[doff + i] = d;
     }
     doff += own + PADS;
   }
+  if (staged) __syncthreads();  // the pool is reused for s below
   // top s_{j0+g}, same range as d_g
   {
     const int own = kTile >> g;
@@ -204,14 +240,14 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
       }
       T e0, o0;
       inv_pair<T, FILT>(sv, dv, e0, o0);
-      out[2 * p] = e0;
-      out[2 * p + 1] = o0;
+      *reinterpret_cast<V*>(out + 2 * p) = V{e0, o0};
     }
     __syncthreads();
   }
   // bufX holds s_{j0} for local [0, kTile)
   T* __restrict__ dst = static_cast<T*>(a.dst);
-  for (int q = tid; q < kTile; q += kNT) dst[base + q] = bufX[q];
+  for (int q = 2 * tid; q < kTile; q += 2 * kNT)
+    *reinterpret_cast<V*>(dst + base + q) = *reinterpret_cast<const V*>(bufX + q);
 }
 
 // ----------------------------------------------------------- whole kernels
@@ -402,7 +438,7 @@ int dwt_fwd_f(const void* x, void* y, int64_t n, int J, hipStream_t st) {
       a.g = P.g;
       a.final_pass = (P.j0 + P.g == J);
       a.write_coarse = (P.j0 == 0);
-      int64_t blocks = P.N / kTile;
+      const int64_t blocks = P.N / kTile;
       hipLaunchKernelGGL((dwt_fwd_tile<T, FILT>), dim3(unsigned(blocks)), dim3(kNT), 0, st, a);
       WH_HIP_CHECK(hipGetLastError());
       src = out;

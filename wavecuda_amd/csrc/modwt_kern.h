@@ -73,17 +73,6 @@ __device__ __forceinline__ ModwtArgs batch_slice(const ModwtArgs& a) {
 
 constexpr int kGmax = 10;
 
-template <typename T>
-struct Vec2;
-template <>
-struct Vec2<double> {
-  using type = double2;
-};
-template <>
-struct Vec2<float> {
-  using type = float2;
-};
-
 __host__ __device__ constexpr int even_up(int v) { return (v + 1) & ~1; }
 
 // Ring geometry in m units.  Forward ring l (l = 0..G-1) holds the input of

@@ -102,6 +102,18 @@ inline int resolve_levels(int64_t n, int nlevels, int L) {
   return nlevels;
 }
 
+// two consecutive elements as one 16-byte (double) or 8-byte (float) access
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+
 // soft/hard rule of src/thresh.cpp:142-157
 template <typename T>
 __device__ __forceinline__ T thresh_coef(T c, T t, int hardness) {
