@@ -19,6 +19,8 @@ headline value is whole-job throughput in Msamples/s.  Alongside:
   smooth        single-signal universal / CV smoothing at n = 2^18;
   differentiable  per-level thresholds fitted by Adam on 512 x 2^14 FP32
                 signals, against CV smoothing;
+  host_c_abi    the reference-compatible RgpuTransform on host arrays
+                (PCIe copies included), LA8 MODWT J = 10, n = 2^24;
   cpu_baseline  the CPU oracle (the reference's serial CPU path restated)
                 timed on this host on a bounded sample of the headline
                 workload.
@@ -290,6 +292,43 @@ def smooth_section():
     return rows
 
 
+def host_section():
+    """The reference-compatible .C entry point RgpuTransform on host arrays
+    (what R's .C call does): LA8 MODWT J = 10 forward then inverse of
+    n = 2^24 FP64, PCIe copies included (x in, 2 n J coefficients out, and
+    back for the inverse)."""
+    import ctypes as C
+
+    import numpy as np
+
+    from wavecuda_amd import _lib as W
+
+    n, J = 1 << 24, 10
+    x = np.random.Generator(np.random.PCG64(3)).standard_normal(n)
+    xm = np.zeros(2 * n * J)
+    xr = np.zeros(n)
+    dp = C.POINTER(C.c_double)
+
+    def ip(v):
+        return C.byref(C.c_int(v))
+
+    def call(xa, xma, sense):
+        W.lib().RgpuTransform(xa.ctypes.data_as(dp), xma.ctypes.data_as(dp), ip(n), ip(sense),
+                              ip(J), ip(1), ip(W.LA8), ip(8))
+    call(x.copy(), xm, 1)  # warm
+    t0 = time.perf_counter()
+    call(x.copy(), xm, 1)
+    t1 = time.perf_counter()
+    call(xr, xm, 0)
+    t2 = time.perf_counter()
+    err = float(np.max(np.abs(xr - x)) / np.max(np.abs(x)))
+    return {"workload": "RgpuTransform (host arrays, as R's .C passes them), LA8 MODWT J=10, "
+                        "n=2^24 FP64", "fwd_ms": round((t1 - t0) * 1e3, 1),
+            "inv_ms": round((t2 - t1) * 1e3, 1),
+            "msamples_per_s_fwd_plus_inv": round(n / (t2 - t0) / 1e6, 1),
+            "reconstruction_relerr": err}
+
+
 def diff_section(steps=300):
     """Per-level soft thresholds fitted by Adam: 512 doppler + noise signals
     (SNR ~ U[1, 10]) of n = 2^14 in FP32, LA8 MODWT J = 8, loss = mean
@@ -474,6 +513,7 @@ def main():
         out["small"] = small_section(max(5, args.steps // 5))
         out["smooth"] = smooth_section()
         out["differentiable"] = diff_section()
+        out["host_c_abi"] = host_section()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(J=J)
     print(json.dumps(out))
