@@ -99,6 +99,15 @@ template <typename T>
 __device__ __forceinline__ void stv(T* p, typename Vec2<T>::type v) {
   *reinterpret_cast<typename Vec2<T>::type*>(p) = v;
 }
+// streaming read: the inverse reads every coefficient once, so its loads
+// should not displace other data (such as the signal) from the caches
+template <typename T>
+__device__ __forceinline__ typename Vec2<T>::type ldv_nt(const T* p) {
+  typename Vec2<T>::type v;
+  v.x = __builtin_nontemporal_load(p);
+  v.y = __builtin_nontemporal_load(p + 1);
+  return v;
+}
 template <typename T>
 __device__ __forceinline__ void stv_nt(T* p, typename Vec2<T>::type v) {
   __builtin_nontemporal_store(v.x, p);
@@ -447,8 +456,8 @@ struct InvCtx {
   __device__ __forceinline__ void load_next(int64_t cc) {
     const int64_t i = gidx(cc + mm);
 #pragma unroll
-    for (int l = 1; l <= G; ++l) pre[l] = ldv(xmod + 2 * n * int64_t(j0 + l - 1) + n + i);
-    pre[0] = ldv(stop + i);
+    for (int l = 1; l <= G; ++l) pre[l] = ldv_nt(xmod + 2 * n * int64_t(j0 + l - 1) + n + i);
+    pre[0] = ldv_nt(stop + i);
   }
   // park the loaded chunk in the rings (slots head_l + W'_l + mm)
   __device__ __forceinline__ void park() {
@@ -645,8 +654,8 @@ struct InvLinCtx {
   __device__ __forceinline__ void load_next(int64_t cc) {
     const int64_t i = (cc + mm) & (n - 1);
 #pragma unroll
-    for (int l = 1; l <= G; ++l) pre[l] = ldv(xmod + 2 * n * int64_t(l - 1) + n + i);
-    pre[0] = ldv(stop + i);
+    for (int l = 1; l <= G; ++l) pre[l] = ldv_nt(xmod + 2 * n * int64_t(l - 1) + n + i);
+    pre[0] = ldv_nt(stop + i);
   }
   // the loaded chunk goes to slots W_l + mm, mm + 1 of every level
   __device__ __forceinline__ void park() {
