@@ -183,3 +183,25 @@ def test_batch_workload_sample():
         assert same_t(t[i].item(), ts), case
         ysg, tsg = D.smooth(xg[o[i]:o[i + 1]].clone(), lv, tt, f, W.UNIV)
         assert torch.equal(ysg, y[o[i]:o[i + 1]]) and tsg == t[i].item(), case
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_batch_workload_all_signals_equal_single(dtype):
+    """Every one of the 8192 signals of the mixed workload equals the
+    single-signal GPU smoothing (wh_smooth, UNIV) bit for bit, thresholds
+    included."""
+    x, lens, filters, ttypes, levels = batch_workload()
+    plan = BatchPlan(lens, filters, ttypes, levels, dtype=dtype)
+    xg = torch.tensor(x, dtype=dtype, device="cuda")
+    y, t = plan.smooth(xg)
+    tc = t.cpu().numpy()
+    o = plan.offsets
+    bad = []
+    for i in range(len(lens)):
+        yi, ti = D.smooth(xg[o[i]:o[i + 1]], int(levels[i]), int(ttypes[i]), int(filters[i]),
+                          W.UNIV)
+        if not (torch.equal(yi, y[o[i]:o[i + 1]]) and ti == tc[i]):
+            bad.append(i)
+            if len(bad) > 5:
+                break
+    assert not bad, f"signals differing from the single-signal path: {bad}"

@@ -105,10 +105,11 @@ __device__ __forceinline__ void batch_modwt_item(const BatchArgs& b, const Batch
   batch_phase();
 }
 
-// LDS elements per ping-pong buffer of the DWT tail (64 KB for both types)
+// LDS elements per ping-pong buffer of the DWT tail (64 KB in FP64, 32 KB
+// in FP32, so that several FP32 workgroups share a CU)
 template <typename T>
 constexpr int batch_lds_elems() {
-  return sizeof(T) == 8 ? 4096 : 8192;
+  return 4096;
 }
 
 // DWT item: levels run on compact sequences, through scratch while longer

@@ -30,6 +30,7 @@ int batch_occ_nt(size_t lds, int* blocks) {
 template <typename T>
 int batch_occupancy_impl(int NT, size_t lds, int* blocks) {
   switch (NT) {
+    case 128: return batch_occ_nt<T, 128>(lds, blocks);
     case 256: return batch_occ_nt<T, 256>(lds, blocks);
     case 512: return batch_occ_nt<T, 512>(lds, blocks);
     default: return set_error(WH_ERR_UNSUPPORTED, "batch: unsupported workgroup size");
@@ -39,6 +40,7 @@ int batch_occupancy_impl(int NT, size_t lds, int* blocks) {
 template <typename T>
 int batch_launch_impl(const BatchLaunch& bl, const BatchArgs& a, hipStream_t st) {
   switch (bl.NT) {
+    case 128: return batch_launch_nt<T, 128>(bl, a, st);
     case 256: return batch_launch_nt<T, 256>(bl, a, st);
     case 512: return batch_launch_nt<T, 512>(bl, a, st);
     default: return set_error(WH_ERR_UNSUPPORTED, "batch: unsupported workgroup size");

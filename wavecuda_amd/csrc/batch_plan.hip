@@ -24,7 +24,7 @@ struct wh_batch {
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
   int64_t stride = 0;         // workspace elements per workgroup
-  size_t lds[2] = {0, 0};     // dynamic LDS for 256 / 512 threads
+  size_t lds[3] = {0, 0, 0};  // dynamic LDS for 128 / 256 / 512 threads
   int NT = 256;
   unsigned grid = 0;
   int tuned = 0;
@@ -35,17 +35,18 @@ struct wh_batch {
 namespace wh {
 namespace {
 
-constexpr int kNTs[2] = {256, 512};
+constexpr int kNTs[3] = {128, 256, 512};
 
 std::mutex g_tune_mu;
 std::map<std::string, std::pair<int, unsigned>> g_tune;  // key -> (NT, grid)
 
-int nt_index(int NT) { return NT == 512 ? 1 : 0; }
+int nt_index(int NT) { return NT == 512 ? 2 : (NT == 256 ? 1 : 0); }
 
 // dynamic LDS of one item for NT threads
 size_t item_lds(int ttype, int filt, int64_t n, int J, int NT, size_t esz) {
   const int L = filter_length(filt);
-  size_t b = (NT == 512) ? block_sel_lds<512>() : block_sel_lds<256>();
+  size_t b = (NT == 512) ? block_sel_lds<512>()
+             : (NT == 256 ? block_sel_lds<256>() : block_sel_lds<128>());
   if (ttype == kDWT) {
     const size_t tile = (esz == 8) ? size_t(batch_lds_elems<double>()) * 8
                                    : size_t(batch_lds_elems<float>()) * 4;
@@ -136,7 +137,7 @@ int wh_batch_create(const int* lens, const int* filters, const int* ttypes, cons
   std::vector<BatchItem> items(static_cast<size_t>(count));
   std::map<int64_t, int> comp;  // (kind, log2 n, J) -> count
   int64_t off = 0, stride = 0;
-  size_t lds[2] = {0, 0};
+  size_t lds[3] = {0, 0, 0};
   for (int i = 0; i < count; ++i) {
     const int64_t n = lens[i];
     const int filt = filters[i], tt = ttypes[i];
@@ -161,7 +162,7 @@ int wh_batch_create(const int* lens, const int* filters, const int* ttypes, cons
     I.factor = std::sqrt(2.0 * std::log(nn));
     off += n;
     stride = std::max<int64_t>(stride, tt == kDWT ? 2 * n : 2 * n * J);
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < 3; ++k)
       lds[k] = std::max(lds[k], item_lds(tt, filt, n, J, kNTs[k], esz));
     int lg = 0;
     while ((int64_t(1) << lg) < n) ++lg;
@@ -183,8 +184,7 @@ int wh_batch_create(const int* lens, const int* filters, const int* ttypes, cons
   p->count = count;
   p->total = off;
   p->stride = (stride + 63) / 64 * 64;
-  p->lds[0] = lds[0];
-  p->lds[1] = lds[1];
+  for (int k = 0; k < 3; ++k) p->lds[k] = lds[k];
   p->key = std::to_string(dtype) + "/" + std::to_string(hardness) + ":";
   for (auto& kv : comp) p->key += std::to_string(kv.first) + "x" + std::to_string(kv.second) + ",";
 

@@ -20,7 +20,7 @@ namespace wh {
 
 constexpr int kSelBits = 11;
 constexpr int kSelBins = 1 << kSelBits;
-constexpr int kSelCap = 4096;   // keys kept in LDS
+constexpr int kSelCap = 2048;   // keys kept in LDS
 constexpr int kSelUnroll = 8;   // loads in flight per thread and pass
 
 __device__ __forceinline__ uint64_t sel_key(double v) {
