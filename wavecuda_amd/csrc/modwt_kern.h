@@ -672,10 +672,42 @@ struct InvLinCtx {
       }
     }
   }
+  // From stride 8 on (lanes then read consecutive slots), a level can give
+  // each of M/4 threads four outputs q0 + j s of one residue class: their
+  // taps q0 + u s, u = -(L-1)..3, are L + 3 slot reads for four outputs
+  // instead of L + 1 for two.  Needs M to be a multiple of 4 s.
+  template <int l>
+  __device__ __forceinline__ void level4() {
+    constexpr int s = 1 << (l - 1);
+    constexpr int W = inv_win(L, l);
+    const int tt = threadIdx.x;
+    if (tt < (M >> 2)) {
+      const int q0 = 4 * s * (tt / s) + (tt % s);
+      const T* b = lds + 2 * (off(l) + W + q0 - (L - 1) * s);
+      T ws[L + 3], wd[L + 3];
+#pragma unroll
+      for (int u = 0; u < L + 3; ++u) {
+        const V v = ldv(b + 2 * u * s);
+        ws[u] = v.x;
+        wd[u] = v.y;
+      }
+      T* o = lds + 2 * (off(l - 1) + inv_win(L, l - 1) + q0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) o[2 * j * s] = synthesis<T, FILT>(ws, wd, L - 1 + j);
+    }
+    __syncthreads();
+  }
+
   template <int l, bool STORE>
   __device__ __forceinline__ void level() {
     constexpr int s = 1 << (l - 1);
     constexpr int W = inv_win(L, l);
+    if constexpr (s >= 8) {
+      if ((M & (4 * s - 1)) == 0) {
+        level4<l>();
+        return;
+      }
+    }
     const int t = mm >> 1;
     const int q0 = (s == 1) ? mm : 2 * s * (t / s) + (t % s);
     // taps of outputs q0 and q0 + s: positions q0 + u s, u = -(L-1)..1
