@@ -205,18 +205,20 @@ def _time_ms(fn, steps, warmup=2):
 
 
 def sweep_section(steps):
-    """Single-signal transforms, FP64, white noise: every filter, DWT at full
-    depth and MODWT J = 10, n = 2^20 and 2^24; forward and inverse timed
-    separately, priced at their algorithmic bytes."""
+    """Single-signal transforms, white noise: every filter, DWT at full depth
+    and MODWT J = 10, n = 2^20 and 2^24 in FP64 and n = 2^24 in FP32;
+    forward and inverse timed separately, priced at their algorithmic
+    bytes."""
     import torch
 
     from wavecuda_amd import _lib as W
     from wavecuda_amd import device as D
 
     rows = []
-    for lg in (20, 24):
+    for lg, dt in ((20, torch.float64), (24, torch.float64), (24, torch.float32)):
         n = 1 << lg
-        x = torch.randn(n, dtype=torch.float64, device="cuda",
+        esz = 8 if dt == torch.float64 else 4
+        x = torch.randn(n, dtype=dt, device="cuda",
                         generator=torch.Generator(device="cuda").manual_seed(lg))
         for f in (W.HAAR, W.D4, W.C6, W.LA8):
             Jd = D.resolve_levels(n, 0, f)
@@ -224,19 +226,21 @@ def sweep_section(steps):
             xr = torch.empty_like(x)
             tf = _time_ms(lambda: D.dwt_fwd(x, Jd, f, out=y), steps)
             ti = _time_ms(lambda: D.dwt_inv(y, Jd, f, out=xr), steps)
-            b = 16 * n
+            b = 2 * esz * n
             rows.append({"transform": "DWT", "filter": W.FILTER_NAMES[f], "n": n, "levels": Jd,
+                         "dtype": "f64" if esz == 8 else "f32",
                          "fwd_ms": round(tf, 4), "inv_ms": round(ti, 4),
                          "fwd_GBs": round(b / tf / 1e6, 1), "inv_GBs": round(b / ti / 1e6, 1),
                          "msamples_per_s": round(n / (tf + ti) / 1e3, 1)})
             del y, xr
             J = 10
-            xm = torch.empty(2 * n * J, dtype=torch.float64, device="cuda")
+            xm = torch.empty(2 * n * J, dtype=dt, device="cuda")
             xr = torch.empty_like(x)
             tf = _time_ms(lambda: D.modwt_fwd(x, J, f, out=xm), steps)
             ti = _time_ms(lambda: D.modwt_inv(xm, n, J, f, out=xr), steps)
-            bf, bi = modwt_bytes_fwd(n, J), modwt_bytes_inv(n, J)
+            bf, bi = modwt_bytes_fwd(n, J, esz), modwt_bytes_inv(n, J, esz)
             rows.append({"transform": "MODWT", "filter": W.FILTER_NAMES[f], "n": n, "levels": J,
+                         "dtype": "f64" if esz == 8 else "f32",
                          "fwd_ms": round(tf, 4), "inv_ms": round(ti, 4),
                          "fwd_GBs": round(bf / tf / 1e6, 1), "inv_GBs": round(bi / ti / 1e6, 1),
                          "msamples_per_s": round(n / (tf + ti) / 1e3, 1)})

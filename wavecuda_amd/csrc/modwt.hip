@@ -62,6 +62,10 @@ std::vector<Group> plan_groups(bool inverse, int64_t n, int J, int L, size_t esz
     int gbest = gmin;
     for (int g = gmin + 1; g <= std::min(J - j0, kGmax); ++g) {
       if (ring_elems(inverse, L, g, G.M) * G.R * esz > budget) break;
+      // the first inverse group keeps linear buffers whose windows move once
+      // per chunk: a window longer than the chunk would cost more to move
+      // than the chunk itself
+      if (inverse && G.R == 1 && inv_win(L, g) > G.M) break;
       int64_t warm = round_up64(int64_t(L - 1) * ((int64_t(1) << g) - 1), G.M);
       if (warm > Pt / 2 && nseg_t > 1) break;
       gbest = g;
