@@ -106,6 +106,16 @@ int wh_dwt_inv(const void* y, void* x, int64_t n, int nlevels, int filter, int d
 int wh_threshold(void* coeffs, int64_t n, int nlevels, int ttype, int filter, int dtype,
                  const wh_thresh* thr, void* stream);
 
+/* Plan tuning for the fused MODWT: times the planner's alternatives (depth
+   of the first fused level group, segments per CU) for this length, level
+   count, filter and element type on scratch buffers of its own (allocated
+   and freed here; about (2J + 2) n elements) and keeps the fastest forward
+   and inverse plans for later calls in this process.  Results do not
+   change: every plan computes each coefficient with the same operations.
+   Synchronous.  No counterpart in the reference, whose GPU MODWT runs one
+   kernel per level (src/haarcuda.cu:1649-1914). */
+int wh_modwt_tune(int64_t n, int nlevels, int filter, int dtype, void* stream);
+
 /* ---- many signals, adjoints ------------------------------------------ */
 
 /* The MODWT of `count` signals of n samples at once (1 <= count <= 65535):

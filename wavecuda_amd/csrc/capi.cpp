@@ -341,6 +341,14 @@ int wh_smooth(void* x, int64_t n, int nlevels, int ttype, int filter, int dtype,
                 tol, static_cast<hipStream_t>(stream));
 }
 
+int wh_modwt_tune(int64_t n, int nlevels, int filter, int dtype, void* stream) {
+  if (!valid_filter(filter)) return set_error(WH_ERR_ARG, "filter must be 1..4");
+  if (!valid_dtype(dtype)) return set_error(WH_ERR_ARG, "dtype must be WH_F64 or WH_F32");
+  const int J = resolve_levels(n, nlevels, filter_length(filter));
+  if (J <= 0) return set_error(WH_ERR_ARG, "invalid length / levels");
+  return modwt_tune(n, J, filter, dtype, static_cast<hipStream_t>(stream));
+}
+
 // Diagnostic: the level-group plan chosen for a MODWT (written to buf).
 int wh_debug_modwt_plan(int inverse, int64_t n, int nlevels, int filter, int dtype, char* buf,
                         int cap) {

@@ -2,6 +2,10 @@
 // launches.  The kernels and their design notes are in modwt_kern.h; they
 // are instantiated per (element type, filter) in modwt_<type>_<filter>.hip.
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
 #include <vector>
 
 #include "modwt_kern.h"
@@ -28,6 +32,27 @@ struct Group {
   size_t lds;
 };
 
+// Planner choices the tuner may change (defaults: the rules below).
+struct PlanKnobs {
+  int g1max = kGmax;                 // deepest first group
+  int first_per_cu = 1;              // segments per CU of the first group
+  int deep_per_cu = kDeepSegPerCu;   // segments per CU of deeper groups
+  bool operator<(const PlanKnobs& o) const {
+    return std::tie(g1max, first_per_cu, deep_per_cu) <
+           std::tie(o.g1max, o.first_per_cu, o.deep_per_cu);
+  }
+};
+
+using KnobKey = std::tuple<int, int64_t, int, int, int>;  // inverse, n, J, filter, element size
+std::mutex g_knob_mu;
+std::map<KnobKey, PlanKnobs> g_knobs;
+
+PlanKnobs knobs_for(bool inverse, int64_t n, int J, int filt, size_t esz) {
+  std::lock_guard<std::mutex> lk(g_knob_mu);
+  auto it = g_knobs.find(KnobKey{inverse ? 1 : 0, n, J, filt, int(esz)});
+  return it == g_knobs.end() ? PlanKnobs{} : it->second;
+}
+
 size_t ring_elems(bool inverse, int L, int g, int M) {
   return inverse ? size_t(2) * inv_off(L, M, g + 1) : size_t(fwd_off(L, M, g));
 }
@@ -39,7 +64,8 @@ int64_t round_up64(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 // and run R = 32 residues; every group fuses as many levels as the LDS
 // budget allows while its warm-up stays at most half a segment (one
 // segment per CU for the first group, kDeepSegPerCu for deeper ones).
-std::vector<Group> plan_groups(bool inverse, int64_t n, int J, int L, size_t esz) {
+std::vector<Group> plan_groups(bool inverse, int64_t n, int J, int L, size_t esz,
+                               const PlanKnobs& k) {
   const DeviceInfo& dev = device_info();
   const size_t budget = std::min<size_t>(dev.max_lds_per_block, 160 * 1024) - 1024;
   const int64_t cus = std::max(1, dev.num_cus);
@@ -54,13 +80,15 @@ std::vector<Group> plan_groups(bool inverse, int64_t n, int J, int L, size_t esz
     const int64_t Nm = n >> j0;
     G.M = int(std::min<int64_t>(2 * G.NT / G.R, Nm));
     const int64_t nrb = B / G.R;
-    const int64_t target = std::max<int64_t>(1, (j0 == 0 ? 1 : kDeepSegPerCu) * cus / nrb);
+    const int64_t target =
+        std::max<int64_t>(1, (j0 == 0 ? k.first_per_cu : k.deep_per_cu) * cus / nrb);
     int64_t nseg_t = 1;
     while (nseg_t * 2 <= target && Nm / (nseg_t * 2) >= G.M) nseg_t *= 2;
     const int64_t Pt = Nm / nseg_t;
     const int gmin = (j0 == 0) ? std::min(J, 5) : 1;
     int gbest = gmin;
-    for (int g = gmin + 1; g <= std::min(J - j0, kGmax); ++g) {
+    const int gcap = (j0 == 0) ? std::max(gmin, std::min(k.g1max, kGmax)) : kGmax;
+    for (int g = gmin + 1; g <= std::min(J - j0, gcap); ++g) {
       if (ring_elems(inverse, L, g, G.M) * G.R * esz > budget) break;
       // the first inverse group keeps linear buffers whose windows move once
       // per chunk: a window longer than the chunk would cost more to move
@@ -136,7 +164,7 @@ template <typename T>
 int modwt_fwd_t(const void* x, void* xmod, int64_t n, int64_t count, int J, int filt, int sp,
                 hipStream_t st) {
   const int L = filter_length(filt);
-  auto groups = plan_groups(false, n, J, L, sizeof(T));
+  auto groups = plan_groups(false, n, J, L, sizeof(T), knobs_for(false, n, J, filt, sizeof(T)));
   const int64_t per = 2 * n * J;
   for (const Group& G : groups) {
     ModwtArgs a = args_of(G, n);
@@ -166,7 +194,7 @@ template <typename T>
 int modwt_inv_t(const void* xmod, void* x, int64_t n, int64_t count, int J, int filt,
                 const wh_thresh* thr, hipStream_t st) {
   const int L = filter_length(filt);
-  auto groups = plan_groups(true, n, J, L, sizeof(T));
+  auto groups = plan_groups(true, n, J, L, sizeof(T), knobs_for(true, n, J, filt, sizeof(T)));
   const int64_t per = 2 * n * J;
   // the S_{j0} handed between groups lives in library scratch (2 n values
   // per signal: two halves used alternately)
@@ -225,10 +253,112 @@ int modwt_inverse_adjoint(const void* gx, void* gxmod, int64_t n, int64_t count,
                          : modwt_fwd_t<double>(gx, gxmod, n, count, J, filt, kStoreAdjoint, st);
 }
 
+namespace {
+
+template <typename T>
+int time_plan(bool inverse, const void* x, void* xmod, void* xr, int64_t n, int J, int filt,
+              hipStream_t st, hipEvent_t e0, hipEvent_t e1, float* ms) {
+  // an inverse is timed right after a forward, as it runs in a pipeline
+  // (the caches then hold what the forward left), a forward after an inverse
+  float best = 1e30f;
+  for (int r = 0; r < 7; ++r) {
+    int rc = inverse ? modwt_fwd_t<T>(x, xmod, n, 1, J, filt, kStoreAll, st)
+                     : modwt_inv_t<T>(xmod, xr, n, 1, J, filt, nullptr, st);
+    if (rc) return rc;
+    WH_HIP_CHECK(hipEventRecord(e0, st));
+    rc = inverse ? modwt_inv_t<T>(xmod, xr, n, 1, J, filt, nullptr, st)
+                 : modwt_fwd_t<T>(x, xmod, n, 1, J, filt, kStoreAll, st);
+    if (rc) return rc;
+    WH_HIP_CHECK(hipEventRecord(e1, st));
+    WH_HIP_CHECK(hipEventSynchronize(e1));
+    float t = 0.f;
+    WH_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+    if (r > 0) best = std::min(best, t);  // the first run is a warm-up
+  }
+  *ms = best;
+  return WH_OK;
+}
+
+template <typename T>
+int tune_t(int64_t n, int J, int filt, hipStream_t st) {
+  const int L = filter_length(filt);
+  const size_t esz = sizeof(T);
+  void *x = nullptr, *xmod = nullptr, *xr = nullptr;
+  auto release = [&] {
+    if (x) (void)hipFree(x);
+    if (xmod) (void)hipFree(xmod);
+    if (xr) (void)hipFree(xr);
+  };
+  if (hipMalloc(&x, size_t(n) * esz) != hipSuccess ||
+      hipMalloc(&xmod, size_t(2) * size_t(n) * size_t(J) * esz) != hipSuccess ||
+      hipMalloc(&xr, size_t(n) * esz) != hipSuccess) {
+    release();
+    return set_error(WH_ERR_NOMEM, "modwt tune: buffer allocation failed");
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = WH_OK;
+  if (hipMemsetAsync(x, 0, size_t(n) * esz, st) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    rc = set_hip_error(hipGetLastError(), "modwt tune: setup");
+  for (int inv = 0; inv < 2 && !rc; ++inv) {
+    const bool inverse = inv == 1;
+    std::map<std::string, PlanKnobs> cands;  // distinct plans only
+    for (int g1 = std::min(J, 5); g1 <= kGmax; ++g1)
+      for (int f : {1, 2})
+        for (int d : {1, 2, 4, 8}) {
+          PlanKnobs k;
+          k.g1max = g1;
+          k.first_per_cu = f;
+          k.deep_per_cu = d;
+          std::string key;
+          for (const Group& G : plan_groups(inverse, n, J, L, esz, k))
+            key += std::to_string(G.g) + "/" + std::to_string(G.nseg) + ";";
+          cands.emplace(key, k);
+        }
+    // the default plan is timed first and another one is taken only when
+    // it is faster by more than the run-to-run spread of these timings
+    // (about 1%), so that noise cannot replace the default by a plan that is
+    // no better
+    constexpr float kMargin = 0.98f;
+    std::vector<PlanKnobs> order{PlanKnobs{}};
+    for (auto& c : cands) order.push_back(c.second);
+    float best = 1e30f;
+    PlanKnobs bk;
+    for (size_t i = 0; i < order.size(); ++i) {
+      {
+        std::lock_guard<std::mutex> lk(g_knob_mu);
+        g_knobs[KnobKey{inv, n, J, filt, int(esz)}] = order[i];
+      }
+      float ms = 0.f;
+      if ((rc = time_plan<T>(inverse, x, xmod, xr, n, J, filt, st, e0, e1, &ms))) break;
+      if (i == 0 || ms < best * kMargin) {
+        best = ms;
+        bk = order[i];
+      }
+    }
+    std::lock_guard<std::mutex> lk(g_knob_mu);
+    if (rc)
+      g_knobs.erase(KnobKey{inv, n, J, filt, int(esz)});
+    else
+      g_knobs[KnobKey{inv, n, J, filt, int(esz)}] = bk;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(st);
+  release();
+  return rc;
+}
+
+}  // namespace
+
+int modwt_tune(int64_t n, int J, int filt, int dtype, hipStream_t st) {
+  return dtype == WH_F32 ? tune_t<float>(n, J, filt, st) : tune_t<double>(n, J, filt, st);
+}
+
 std::string modwt_plan_string(bool inverse, int64_t n, int J, int filt, int dtype) {
   const int L = filter_length(filt);
   size_t esz = dtype == WH_F32 ? 4 : 8;
-  auto gs = plan_groups(inverse, n, J, L, esz);
+  auto gs = plan_groups(inverse, n, J, L, esz, knobs_for(inverse, n, J, filt, esz));
   std::string s;
   for (auto& G : gs) {
     char buf[256];

@@ -53,6 +53,9 @@ int modwt_forward_adjoint(const void* gxmod, void* gx, int64_t n, int64_t count,
 int thresh_grad(const void* c, const void* gy, void* gc, double* glam, int64_t n, int64_t count,
                 int J, int ttype, int dtype, const wh_thresh* thr, hipStream_t st);
 std::string modwt_plan_string(bool inverse, int64_t n, int J, int filt, int dtype);
+// time the planner's alternatives for (n, J, filter, element type) and keep
+// the fastest forward and inverse plans for the rest of the process
+int modwt_tune(int64_t n, int J, int filt, int dtype, hipStream_t st);
 
 int dwt_forward(const void* x, void* y, int64_t n, int J, int filt, int dtype, hipStream_t st);
 int dwt_inverse(const void* y, void* x, int64_t n, int J, int filt, int dtype,

@@ -140,6 +140,14 @@ def dwt_inv(y: torch.Tensor, nlevels: int = 0, filter="LA8", thresh=None,
     return out
 
 
+def modwt_tune(n: int, nlevels: int, filter="LA8", dtype=torch.float64) -> None:
+    """Time the fused MODWT's plan alternatives for this shape and keep the
+    fastest for later calls (synchronous)."""
+    f = filter_code(filter)
+    W.check(W.lib().wh_modwt_tune(int(n), resolve_levels(n, nlevels, f), f, _DT[dtype],
+                                  _stream()), "wh_modwt_tune")
+
+
 def modwt_plan(inverse: bool, n: int, nlevels: int, filter="LA8", dtype=torch.float64) -> str:
     buf = C.create_string_buffer(4096)
     rc = W.lib().wh_debug_modwt_plan(1 if inverse else 0, n, nlevels, filter_code(filter),
