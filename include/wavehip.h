@@ -13,7 +13,9 @@
  * 2. `wh_*` functions on device pointers with a status return, an
  *    explicit element type, 64-bit lengths and a caller stream (a
  *    hipStream_t passed as void*; NULL = the null stream).  Work is
- *    enqueued asynchronously unless stated otherwise.
+ *    enqueued asynchronously unless stated otherwise.  Device buffers of
+ *    the transforms and thresholds must be aligned to two elements (16
+ *    bytes for FP64, 8 for FP32; hipMalloc gives 256); WH_ERR_ARG otherwise.
  *
  * Codes are the reference's: sense FWD=1/BWD=0, ttype DWT=0/MODWT=1
  * (time ordered), filter Haar=1/D4=2/C6=3/LA8=4, hardness HARD=0/SOFT=1,

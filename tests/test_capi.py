@@ -38,3 +38,21 @@ def test_no_device_needed_for_pure_queries():
     assert L.wh_resolve_levels(256, 7, 4) == 0
     assert L.wh_resolve_levels(8, 0, 3) == 1
     assert isinstance(L.wh_last_error(), bytes)
+
+
+def test_argument_checks_before_any_device_call():
+    """Argument errors come back as WH_ERR_ARG before the library touches a
+    device: null pointers, codes out of range, and buffers not aligned to
+    two elements (the kernels move element pairs with one vector access)."""
+    import ctypes as C
+
+    L = W.lib()
+    good, odd8, odd4 = C.c_void_p(1 << 20), C.c_void_p((1 << 20) + 8), C.c_void_p((1 << 20) + 4)
+    assert L.wh_modwt_fwd(None, good, 1024, 4, 4, 0, None) == 1
+    assert L.wh_modwt_fwd(good, good, 1024, 4, 9, 0, None) == 1
+    assert L.wh_modwt_fwd(good, good, 1000, 4, 4, 0, None) == 1
+    assert L.wh_modwt_fwd(good, good, 1024, 4, 4, 7, None) == 1
+    assert L.wh_modwt_fwd(odd8, good, 1024, 4, 4, 0, None) == 1  # FP64 pairs: 16 bytes
+    assert b"aligned" in L.wh_last_error()
+    assert L.wh_dwt_inv(good, odd4, 1024, 4, 2, 1, None, None) == 1  # FP32 pairs: 8 bytes
+    assert b"aligned" in L.wh_last_error()

@@ -3,6 +3,7 @@
 // graph.  The kernel is in batch_kern.h, instantiated in batch_<type>.hip.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <map>
 #include <mutex>
 #include <string>
@@ -54,7 +55,7 @@ size_t item_lds(int ttype, int filt, int64_t n, int J, int NT, size_t esz) {
   }
   const int M = int(std::min<int64_t>(2 * NT, n));
   const size_t fwd = size_t(fwd_off(L, M, J)) * esz;
-  const size_t inv = size_t(2) * inv_off(L, M, J + 1) * esz;
+  const size_t inv = size_t(2) * lin_off(L, M, J + 1) * esz;
   return std::max(b, std::max(fwd, inv));
 }
 
@@ -237,13 +238,23 @@ void wh_batch_destroy(wh_batch* p) {
   delete p;
 }
 
-int wh_batch_smooth(wh_batch* p, const void* x, void* y, double* thresh_out, void* stream) {
+// the packed signals start at even offsets; the kernels move element pairs
+// with single vector accesses, so the buffers must be aligned to two elements
+static int check_io(const wh_batch* p, const void* x, const void* y) {
   if (!p || !x || !y) return set_error(WH_ERR_ARG, "null pointer");
+  const uintptr_t pair = p->dtype == WH_F64 ? 16 : 8;
+  if ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y)) & (pair - 1))
+    return set_error(WH_ERR_ARG, "device buffers must be aligned to two elements (16 bytes FP64, 8 bytes FP32)");
+  return WH_OK;
+}
+
+int wh_batch_smooth(wh_batch* p, const void* x, void* y, double* thresh_out, void* stream) {
+  if (int rc = check_io(p, x, y)) return rc;
   return run(p, p->NT, p->grid, x, y, thresh_out, static_cast<hipStream_t>(stream));
 }
 
 int wh_batch_tune(wh_batch* p, const void* x, void* y, double* thresh_out, void* stream) {
-  if (!p || !x || !y) return set_error(WH_ERR_ARG, "null pointer");
+  if (int rc = check_io(p, x, y)) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int cus = std::max(1, device_info().num_cus);
   struct Cand {
@@ -299,7 +310,7 @@ int wh_batch_tune(wh_batch* p, const void* x, void* y, double* thresh_out, void*
 }
 
 int wh_batch_capture(wh_batch* p, const void* x, void* y, double* thresh_out) {
-  if (!p || !x || !y) return set_error(WH_ERR_ARG, "null pointer");
+  if (int rc = check_io(p, x, y)) return rc;
   drop_graph(p);
   hipStream_t cs;
   WH_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -116,6 +117,11 @@ bool valid_dtype(int t) { return t == WH_F64 || t == WH_F32; }
 int check_common(const void* a, const void* b, int64_t n, int nlevels, int filter, int dtype,
                  int* J) {
   if (!a || !b) return set_error(WH_ERR_ARG, "null pointer");
+  if (!valid_dtype(dtype)) return set_error(WH_ERR_ARG, "dtype must be WH_F64 or WH_F32");
+  // the kernels move element pairs with single vector accesses
+  const uintptr_t pair = dtype == WH_F64 ? 16 : 8;
+  if ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & (pair - 1))
+    return set_error(WH_ERR_ARG, "device buffers must be aligned to two elements (16 bytes FP64, 8 bytes FP32)");
   if (!valid_filter(filter)) return set_error(WH_ERR_ARG, "filter must be 1 (Haar), 2 (D4), 3 (C6) or 4 (LA8)");
   if (!valid_dtype(dtype)) return set_error(WH_ERR_ARG, "dtype must be WH_F64 or WH_F32");
   *J = resolve_levels(n, nlevels, filter_length(filter));

@@ -2,6 +2,8 @@
 // launches.  The kernels and their design notes are in modwt_kern.h; they
 // are instantiated per (element type, filter) in modwt_<type>_<filter>.hip.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <string>
@@ -37,9 +39,11 @@ struct PlanKnobs {
   int g1max = kGmax;                 // deepest first group
   int first_per_cu = 1;              // segments per CU of the first group
   int deep_per_cu = kDeepSegPerCu;   // segments per CU of deeper groups
+  int first_nt = kNT1;               // threads of the first inverse group (512, or 256:
+                                     // two workgroups per CU in half the LDS)
   bool operator<(const PlanKnobs& o) const {
-    return std::tie(g1max, first_per_cu, deep_per_cu) <
-           std::tie(o.g1max, o.first_per_cu, o.deep_per_cu);
+    return std::tie(g1max, first_per_cu, deep_per_cu, first_nt) <
+           std::tie(o.g1max, o.first_per_cu, o.deep_per_cu, o.first_nt);
   }
 };
 
@@ -53,8 +57,11 @@ PlanKnobs knobs_for(bool inverse, int64_t n, int J, int filt, size_t esz) {
   return it == g_knobs.end() ? PlanKnobs{} : it->second;
 }
 
-size_t ring_elems(bool inverse, int L, int g, int M) {
-  return inverse ? size_t(2) * inv_off(L, M, g + 1) : size_t(fwd_off(L, M, g));
+// LDS elements of one residue's buffers (the first inverse group keeps the
+// linear buffers of InvLinCtx, deeper inverse groups rings)
+size_t ring_elems(bool inverse, int L, int g, int M, int R) {
+  if (!inverse) return size_t(fwd_off(L, M, g));
+  return size_t(2) * (R == 1 ? lin_off(L, M, g + 1) : inv_off(L, M, g + 1));
 }
 
 int64_t round_up64(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
@@ -76,7 +83,9 @@ std::vector<Group> plan_groups(bool inverse, int64_t n, int J, int L, size_t esz
     G.j0 = j0;
     const int64_t B = int64_t(1) << j0;
     G.R = (j0 == 0) ? 1 : kRdeep;
-    G.NT = (G.R == 1) ? kNT1 : kNTR;
+    G.NT = (G.R == 1) ? (inverse ? k.first_nt : kNT1) : kNTR;
+    // a 256-thread first inverse group plans for two workgroups per CU
+    const size_t gbudget = (G.R == 1 && G.NT < kNT1) ? budget / 2 : budget;
     const int64_t Nm = n >> j0;
     G.M = int(std::min<int64_t>(2 * G.NT / G.R, Nm));
     const int64_t nrb = B / G.R;
@@ -89,11 +98,14 @@ std::vector<Group> plan_groups(bool inverse, int64_t n, int J, int L, size_t esz
     int gbest = gmin;
     const int gcap = (j0 == 0) ? std::max(gmin, std::min(k.g1max, kGmax)) : kGmax;
     for (int g = gmin + 1; g <= std::min(J - j0, gcap); ++g) {
-      if (ring_elems(inverse, L, g, G.M) * G.R * esz > budget) break;
+      if (ring_elems(inverse, L, g, G.M, G.R) * G.R * esz > gbudget) break;
       // the first inverse group keeps linear buffers whose windows move once
       // per chunk: a window longer than the chunk would cost more to move
       // than the chunk itself
       if (inverse && G.R == 1 && inv_win(L, g) > G.M) break;
+      // the linear buffers' pair mapping takes outputs q and q + s of a
+      // chunk, so the group's deepest stride 2^{g-1} must fit twice
+      if (inverse && G.R == 1 && (int64_t(1) << g) > G.M) break;
       int64_t warm = round_up64(int64_t(L - 1) * ((int64_t(1) << g) - 1), G.M);
       if (warm > Pt / 2 && nseg_t > 1) break;
       gbest = g;
@@ -107,7 +119,7 @@ std::vector<Group> plan_groups(bool inverse, int64_t n, int J, int L, size_t esz
     G.nseg = int(nseg);
     G.P = Nm / nseg;
     G.nchunks = int((G.P + G.warm) / G.M);
-    G.lds = ring_elems(inverse, L, G.g, G.M) * G.R * esz;
+    G.lds = ring_elems(inverse, L, G.g, G.M, G.R) * G.R * esz;
     gs.push_back(G);
     j0 += G.g;
   }
@@ -132,6 +144,8 @@ int launch(bool inverse, int filt, const Group& G, const ModwtArgs& a, int64_t c
   // 2L; it is LDS-bound)
   gl.map = inverse ? 2 : 0;
   if (gl.blocks > 0x7fffffff) return set_error(WH_ERR_ARG, "modwt: grid too large");
+  if (inverse && G.R == 1 && (int64_t(1) << G.g) > G.M)
+    return set_error(WH_ERR_UNSUPPORTED, "modwt: inverse group deeper than its chunk allows");
   switch (filt) {
     case kHAAR: return modwt_launch<T, kHAAR>(inverse, gl, a, st);
     case kD4: return modwt_launch<T, kD4>(inverse, gl, a, st);
@@ -303,23 +317,30 @@ int tune_t(int64_t n, int J, int filt, hipStream_t st) {
   for (int inv = 0; inv < 2 && !rc; ++inv) {
     const bool inverse = inv == 1;
     std::map<std::string, PlanKnobs> cands;  // distinct plans only
-    for (int g1 = std::min(J, 5); g1 <= kGmax; ++g1)
-      for (int f : {1, 2})
-        for (int d : {1, 2, 4, 8}) {
-          PlanKnobs k;
-          k.g1max = g1;
-          k.first_per_cu = f;
-          k.deep_per_cu = d;
-          std::string key;
-          for (const Group& G : plan_groups(inverse, n, J, L, esz, k))
-            key += std::to_string(G.g) + "/" + std::to_string(G.nseg) + ";";
-          cands.emplace(key, k);
-        }
+    for (int nt : {kNT1, kNT1 / 2}) {
+      if (nt != kNT1 && !inverse) continue;
+      for (int g1 = std::min(J, 5); g1 <= kGmax; ++g1)
+        for (int f : {1, 2, 4})
+          for (int d : {1, 2, 4, 8}) {
+            PlanKnobs k;
+            k.g1max = g1;
+            k.first_per_cu = f;
+            k.deep_per_cu = d;
+            k.first_nt = nt;
+            std::string key;
+            for (const Group& G : plan_groups(inverse, n, J, L, esz, k))
+              key += std::to_string(G.g) + "/" + std::to_string(G.nseg) + "/" +
+                     std::to_string(G.NT) + ";";
+            cands.emplace(key, k);
+          }
+    }
     // the default plan is timed first and another one is taken only when
     // it is faster by more than the run-to-run spread of these timings
     // (about 1%), so that noise cannot replace the default by a plan that is
     // no better
     constexpr float kMargin = 0.98f;
+    const char* env = std::getenv("WH_TUNE_LOG");  // candidate timings to stderr
+    const bool log = env && env[0] == '1';
     std::vector<PlanKnobs> order{PlanKnobs{}};
     for (auto& c : cands) order.push_back(c.second);
     float best = 1e30f;
@@ -331,6 +352,9 @@ int tune_t(int64_t n, int J, int filt, hipStream_t st) {
       }
       float ms = 0.f;
       if ((rc = time_plan<T>(inverse, x, xmod, xr, n, J, filt, st, e0, e1, &ms))) break;
+      if (log)
+        std::fprintf(stderr, "wh_modwt_tune %s %.4f ms %s\n", inverse ? "inv" : "fwd", ms,
+                     modwt_plan_string(inverse, n, J, filt, esz == 4 ? WH_F32 : WH_F64).c_str());
       if (i == 0 || ms < best * kMargin) {
         best = ms;
         bk = order[i];

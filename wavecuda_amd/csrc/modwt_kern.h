@@ -91,6 +91,27 @@ __host__ __device__ inline int inv_off(int L, int M, int l) {
   return (l - 1) * M + inv_win_sum(L, l);
 }
 
+// Linear buffers of the first inverse group (InvLinCtx).  Level l holds the
+// positions q in [-W_l, M) of the chunk, p = q + W_l.  Levels of stride
+// s = 2^{l-1} <= 8 with M >= 4 s are "planar": position p lives in plane
+// 4 (p mod s) + (p / s mod 4) at index p / (4 s), so the four-outputs-per-
+// thread mapping reads consecutive slots across lanes (no LDS bank
+// conflicts).  Other levels keep position order (slot p).
+__host__ __device__ constexpr bool lin_planar(int M, int l) {
+  return l <= 4 && M >= (4 << (l - 1));
+}
+__host__ __device__ inline int lin_plane(int L, int M, int l) {  // slots per plane
+  return ((inv_win(L, l) + M) / (1 << (l - 1)) + 3) / 4;
+}
+__host__ __device__ inline int lin_size(int L, int M, int l) {
+  return lin_planar(M, l) ? (4 << (l - 1)) * lin_plane(L, M, l) : inv_win(L, l) + M;
+}
+__host__ __device__ inline int lin_off(int L, int M, int l) {  // levels 1..l-1
+  int o = 0;
+  for (int i = 1; i < l; ++i) o += lin_size(L, M, i);
+  return o;
+}
+
 template <typename T>
 __device__ __forceinline__ typename Vec2<T>::type ldv(const T* p) {
   return *reinterpret_cast<const typename Vec2<T>::type*>(p);
@@ -101,17 +122,18 @@ __device__ __forceinline__ void stv(T* p, typename Vec2<T>::type v) {
 }
 // streaming read: the inverse reads every coefficient once, so its loads
 // should not displace other data (such as the signal) from the caches
+// (one vector access per pair: p is aligned to two elements)
 template <typename T>
 __device__ __forceinline__ typename Vec2<T>::type ldv_nt(const T* p) {
-  typename Vec2<T>::type v;
-  v.x = __builtin_nontemporal_load(p);
-  v.y = __builtin_nontemporal_load(p + 1);
-  return v;
+  typedef T ev2 __attribute__((ext_vector_type(2)));
+  const ev2 r = __builtin_nontemporal_load(reinterpret_cast<const ev2*>(p));
+  return {r.x, r.y};
 }
 template <typename T>
 __device__ __forceinline__ void stv_nt(T* p, typename Vec2<T>::type v) {
-  __builtin_nontemporal_store(v.x, p);
-  __builtin_nontemporal_store(v.y, p + 1);
+  typedef T ev2 __attribute__((ext_vector_type(2)));
+  const ev2 r = {v.x, v.y};
+  __builtin_nontemporal_store(r, reinterpret_cast<ev2*>(p));
 }
 
 // analysis outputs of taps w[0..L-1] and w[1..L] (reference term order)
@@ -423,6 +445,7 @@ struct InvCtx {
   int64_t n, Nm, r0, c;
   int head[G + 1];  // head of ring l (1..G): slot of position c - W'_l
   V pre[G + 1];     // next chunk: D_l (l = 1..G) and S_G (index 0)
+  T thv[G + 1];     // threshold of level l, negative where none applies
 
   __device__ __forceinline__ int64_t gidx(int64_t m) const {
     m &= (Nm - 1);
@@ -466,7 +489,7 @@ struct InvCtx {
       const int Q = M + inv_win(L, l);
       int so = head[l] + inv_win(L, l) + mm;
       if (so >= Q) so -= Q;
-      const V d = thr2(pre[l], thr(l));
+      const V d = thr2(pre[l], thv[l]);
       if constexpr (kInter) {
         // slots so, so+1 (so even, Q even: no wrap between them)
         T* p = lds + sidx(l, so);
@@ -589,6 +612,8 @@ __device__ __forceinline__ void modwt_inv_body(const ModwtArgs& a, int64_t rb, i
   const int64_t cbot = seg * a.P - a.warm;
 #pragma unroll
   for (int l = 0; l <= G; ++l) C.head[l] = 0;
+#pragma unroll
+  for (int l = 1; l <= G; ++l) C.thv[l] = C.thr(l);  // read once, outside the chunk loop
 
   // prologue: chunk 0 of every ring
   C.load_next(cbot);
@@ -617,14 +642,15 @@ __device__ __forceinline__ void modwt_inv_body(const ModwtArgs& a, int64_t rb, i
 
 // ------------------------------------------------- inverse, linear buffers
 // The R = 1 group with the residue-class mapping, but each level keeps a
-// linear buffer instead of a ring: slots [0, W_l) hold the window (the
-// W_l positions before the chunk) and [W_l, W_l + M) the chunk, so
-// position c + q sits at slot W_l + q and every tap of a thread is a
-// compile-time offset from one per-level base address -- no wrap
-// arithmetic per tap.  At the end of a chunk the last W_l slots of the
-// chunk region move to [0, W_l) (through registers, since the regions
-// overlap when W_l > M) before the next chunk is parked behind them.
-// Slots interleave (S, D) as in the ring version.
+// linear buffer instead of a ring: it holds the W_l positions before the
+// chunk (the window) and the chunk's M positions, so every tap of a thread
+// is a fixed offset from a per-thread base address -- no wrap arithmetic
+// per tap.  At the end of a chunk the last W_l positions move to the window
+// (through registers, since the regions overlap when W_l > M) before the
+// next chunk is parked behind them.  Slots interleave (S, D).  Levels of
+// stride up to 8 use the planar slot order of lin_planar (conflict-free
+// reads for the four-output mapping); deeper levels keep position order,
+// where lanes of one residue class already read consecutive slots.
 template <typename T, int FILT, int G, int NT>
 struct InvLinCtx {
   using V = typename Vec2<T>::type;
@@ -634,11 +660,45 @@ struct InvLinCtx {
   const T* xmod;
   const T* stop;
   T* out;
-  int M, mm;
+  int M, mm, lgM;
   int64_t n, c;
-  V pre[G + 1];  // next chunk: D_l (l = 1..G) and S_G (index 0)
+  int64_t cst;      // where this chunk's outputs go (c; a warm-up chunk stores to the
+                    // segment's first chunk, whose own outputs later replace them)
+  int offs[G + 1];  // first slot of level l's buffer
+  int ps[5];        // slots per plane of the planar levels 1..4, 0 if not planar
+  V pre[G + 1];     // next chunk: D_l (l = 1..G) and S_G (index 0)
+  T thv[G + 1];     // threshold of level l, negative where none applies
 
-  __device__ __forceinline__ int off(int l) const { return inv_off(L, M, l); }
+  __device__ __forceinline__ void init_layout() {
+    lgM = 31 - __clz(M);
+    int o = 0;
+#pragma unroll
+    for (int l = 1; l <= G; ++l) {
+      offs[l] = o;
+      o += lin_size(L, M, l);
+    }
+#pragma unroll
+    for (int l = 1; l <= 4; ++l) ps[l] = lin_planar(M, l) ? lin_plane(L, M, l) : 0;
+#pragma unroll
+    for (int l = 1; l <= G; ++l) thv[l] = thr(l);  // read once, outside the chunk loop
+  }
+  template <int l>
+  __device__ __forceinline__ bool planar() const {
+    if constexpr (l <= 4) return ps[l] != 0;
+    return false;
+  }
+  // slot of chunk position q in [-W_l, M) of level l
+  template <int l>
+  __device__ __forceinline__ int slot(int q) const {
+    const int p = q + inv_win(L, l);
+    if constexpr (l <= 4) {
+      if (ps[l]) {
+        const int r = p & ((1 << (l - 1)) - 1), cc = p >> (l - 1);
+        return offs[l] + ((r << 2) | (cc & 3)) * ps[l] + (cc >> 2);
+      }
+    }
+    return offs[l] + p;
+  }
   __device__ __forceinline__ T thr(int l) const {
     const int lev = l - 1;
     if (!a->th_on || lev < a->minlevel || lev > a->maxlevel) return T(-1);
@@ -657,13 +717,23 @@ struct InvLinCtx {
     for (int l = 1; l <= G; ++l) pre[l] = ldv_nt(xmod + 2 * n * int64_t(l - 1) + n + i);
     pre[0] = ldv_nt(stop + i);
   }
-  // the loaded chunk goes to slots W_l + mm, mm + 1 of every level
-  __device__ __forceinline__ void park() {
-#pragma unroll
-    for (int l = 1; l <= G; ++l) {
-      T* p = lds + 2 * (off(l) + inv_win(L, l) + mm);
-      const V d = thr2(pre[l], thr(l));
-      if (l == G) {
+  // the loaded chunk goes to positions mm, mm + 1 of every level
+  template <int l>
+  __device__ __forceinline__ void park_level() {
+    const V d = thr2(pre[l], thv[l]);
+    if (planar<l>()) {
+      T* p0 = lds + 2 * slot<l>(mm);
+      T* p1 = lds + 2 * slot<l>(mm + 1);
+      if constexpr (l == G) {
+        stv(p0, V{pre[0].x, d.x});
+        stv(p1, V{pre[0].y, d.y});
+      } else {
+        p0[1] = d.x;
+        p1[1] = d.y;
+      }
+    } else {
+      T* p = lds + 2 * slot<l>(mm);
+      if constexpr (l == G) {
         stv(p, V{pre[0].x, d.x});
         stv(p + 2, V{pre[0].y, d.y});
       } else {
@@ -671,19 +741,61 @@ struct InvLinCtx {
         p[3] = d.y;
       }
     }
+    if constexpr (l < G) park_level<l + 1>();
   }
-  // From stride 8 on (lanes then read consecutive slots), a level can give
-  // each of M/4 threads four outputs q0 + j s of one residue class: their
-  // taps q0 + u s, u = -(L-1)..3, are L + 3 slot reads for four outputs
-  // instead of L + 1 for two.  Needs M to be a multiple of 4 s.
+  __device__ __forceinline__ void park() { park_level<1>(); }
+
+  // S_{l-1} at chunk position q (level l > 1) or the output x (level 1)
+  template <int l>
+  __device__ __forceinline__ void put(int q, T e) {
+    lds[2 * slot<l - 1>(q)] = e;
+  }
+
+  // Planar level: each of M/4 threads takes four outputs r + s (4j + i),
+  // i = 0..3, of residue class r; their taps r + s (4j + u + e), u = 0..L+2,
+  // are L + 3 slot reads for four outputs.  Lanes of one class read
+  // consecutive slots of one plane.
+  template <int l>
+  __device__ __forceinline__ void level4p() {
+    constexpr int s = 1 << (l - 1);
+    constexpr int e = inv_win(L, l) / s - (L - 1);  // 1 where the window was rounded up
+    const int tt = threadIdx.x;
+    if (tt < (M >> 2)) {
+      const int sh = lgM - l - 1;  // log2 of the threads per class, M / (4 s)
+      const int r = tt >> sh, j = tt & ((1 << sh) - 1);
+      const T* b = lds + 2 * (offs[l] + (r << 2) * ps[l] + j);
+      T ws[L + 3], wd[L + 3];
+#pragma unroll
+      for (int u = 0; u < L + 3; ++u) {
+        const int cu = u + e;
+        const V v = ldv(b + 2 * ((cu & 3) * ps[l] + (cu >> 2)));
+        ws[u] = v.x;
+        wd[u] = v.y;
+      }
+      T o[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) o[i] = synthesis<T, FILT>(ws, wd, L - 1 + i);
+      if constexpr (l > 1) {
+        const int q0 = r + 4 * s * j;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) put<l>(q0 + i * s, o[i]);
+      } else {
+        stv_nt(out + cst + 4 * j, V{o[0], o[1]});
+        stv_nt(out + cst + 4 * j + 2, V{o[2], o[3]});
+      }
+    }
+    __syncthreads();
+  }
+
+  // Position-ordered level of stride 16 or more: the same four-output
+  // mapping with lanes over consecutive residues; needs M a multiple of 4 s.
   template <int l>
   __device__ __forceinline__ void level4() {
     constexpr int s = 1 << (l - 1);
-    constexpr int W = inv_win(L, l);
     const int tt = threadIdx.x;
     if (tt < (M >> 2)) {
       const int q0 = 4 * s * (tt / s) + (tt % s);
-      const T* b = lds + 2 * (off(l) + W + q0 - (L - 1) * s);
+      const T* b = lds + 2 * slot<l>(q0 - (L - 1) * s);
       T ws[L + 3], wd[L + 3];
 #pragma unroll
       for (int u = 0; u < L + 3; ++u) {
@@ -691,27 +803,32 @@ struct InvLinCtx {
         ws[u] = v.x;
         wd[u] = v.y;
       }
-      T* o = lds + 2 * (off(l - 1) + inv_win(L, l - 1) + q0);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) o[2 * j * s] = synthesis<T, FILT>(ws, wd, L - 1 + j);
+      for (int j = 0; j < 4; ++j) put<l>(q0 + j * s, synthesis<T, FILT>(ws, wd, L - 1 + j));
     }
     __syncthreads();
   }
 
-  template <int l, bool STORE>
+  template <int l>
   __device__ __forceinline__ void level() {
     constexpr int s = 1 << (l - 1);
-    constexpr int W = inv_win(L, l);
-    if constexpr (s >= 8) {
+    if constexpr (l <= 4) {
+      if (planar<l>()) {
+        level4p<l>();
+        return;
+      }
+    }
+    if constexpr (s >= 16) {
       if ((M & (4 * s - 1)) == 0) {
         level4<l>();
         return;
       }
     }
+    // small chunks: two outputs q0, q0 + s of one residue class per thread,
+    // taps q0 + u s, u = -(L-1)..1
     const int t = mm >> 1;
     const int q0 = (s == 1) ? mm : 2 * s * (t / s) + (t % s);
-    // taps of outputs q0 and q0 + s: positions q0 + u s, u = -(L-1)..1
-    const T* b = lds + 2 * (off(l) + W + q0 - (L - 1) * s);
+    const T* b = lds + 2 * slot<l>(q0 - (L - 1) * s);
     T ws[L + 1], wd[L + 1];
 #pragma unroll
     for (int u = 0; u <= L; ++u) {
@@ -722,22 +839,22 @@ struct InvLinCtx {
     const T e0 = synthesis<T, FILT>(ws, wd, L - 1);
     const T e1 = synthesis<T, FILT>(ws, wd, L);
     if constexpr (l > 1) {
-      T* o = lds + 2 * (off(l - 1) + inv_win(L, l - 1) + q0);
-      o[0] = e0;
-      o[2 * s] = e1;
-    } else if constexpr (STORE) {
-      stv_nt(out + c + q0, V{e0, e1});
+      put<l>(q0, e0);
+      put<l>(q0 + s, e1);
+    } else {
+      stv_nt(out + cst + q0, V{e0, e1});
     }
     __syncthreads();
   }
-  template <int l, bool STORE>
+  template <int l>
   __device__ __forceinline__ void levels() {
     if constexpr (l >= 1) {
-      level<l, STORE>();
-      levels<l - 1, STORE>();
+      level<l>();
+      levels<l - 1>();
     }
   }
-  // move slots [M, M + W_l) to [0, W_l) for every level, then park
+  // move positions [M - W_l, M) to [-W_l, 0) for every level, then park
+  // the registers of buffer B
   template <int l>
   __device__ __forceinline__ void shift_and_park() {
     if constexpr (l > G) {
@@ -746,19 +863,23 @@ struct InvLinCtx {
     } else {
       constexpr int W = inv_win(L, l);
       constexpr int K = (W + NT - 1) / NT;
+      // a position and the one M later share a plane (M is a multiple of
+      // 4 s) or are M slots apart
+      const int dist = planar<l>() ? (M >> (l + 1)) : M;
       V tmp[K];
-      const int o = off(l);
+      int dst[K];
       const int tid = threadIdx.x;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         const int i = tid + k * NT;
-        if (i < W) tmp[k] = ldv(lds + 2 * (o + M + i));
+        dst[k] = slot<l>(i - W);
+        if (i < W) tmp[k] = ldv(lds + 2 * (dst[k] + dist));
       }
       shift_and_park<l + 1>();
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         const int i = tid + k * NT;
-        if (i < W) stv(lds + 2 * (o + i), tmp[k]);
+        if (i < W) stv(lds + 2 * dst[k], tmp[k]);
       }
     }
   }
@@ -776,24 +897,24 @@ __device__ __forceinline__ void modwt_inv_lin_body(const ModwtArgs& a, int64_t s
   C.M = a.M;
   C.n = a.n;
   C.mm = (2 * int(threadIdx.x)) % a.M;
+  C.init_layout();
   const int64_t cbot = seg * a.P - a.warm;
 
+  // chunk k + 1's coefficients are loaded while chunk k is computed (the
+  // positions wrap harmlessly past the last chunk).  Warm-up chunks store
+  // their (discarded) outputs where the segment's first chunk later stores
+  // its own, so every store is unconditional and one loop serves both.
+  const int64_t M = a.M;
   C.load_next(cbot);
   C.park();
   __syncthreads();
   const int nwarm = a.warm / a.M;
-  int k = 0;
-  for (; k < nwarm; ++k) {
-    C.c = cbot + int64_t(k) * a.M;
-    C.load_next(C.c + a.M);
-    C.template levels<G, false>();
-    C.template shift_and_park<1>();
-    __syncthreads();
-  }
-  for (; k < a.nchunks; ++k) {
-    C.c = cbot + int64_t(k) * a.M;
-    C.load_next(C.c + a.M);  // wraps harmlessly on the last chunk
-    C.template levels<G, true>();
+  const int64_t first = cbot + a.warm;
+  for (int k = 0; k < a.nchunks; ++k) {
+    C.c = cbot + int64_t(k) * M;
+    C.load_next(C.c + M);
+    C.cst = k < nwarm ? first : C.c;
+    C.template levels<G>();
     C.template shift_and_park<1>();
     __syncthreads();
   }

@@ -46,9 +46,26 @@ int modwt_launch_g(bool inverse, const GroupLaunch& gl, const ModwtArgs& a, hipS
   }
 }
 
+// first inverse group with 256 threads (two workgroups per CU)
+template <typename T, int FILT, int G>
+int modwt_launch_inv256(const GroupLaunch& gl, const ModwtArgs& a, hipStream_t st) {
+  if constexpr (G > kGmax) {
+    return set_error(WH_ERR_UNSUPPORTED, "modwt: group depth out of range");
+  } else {
+    if (gl.G != G) return modwt_launch_inv256<T, FILT, G + 1>(gl, a, st);
+    auto kern = modwt_inv_k<T, FILT, 1, 256, G, 2>;
+    modwt_allow_lds(kern);
+    hipLaunchKernelGGL(kern, dim3(unsigned(gl.blocks), unsigned(gl.count)), dim3(256), gl.lds, st,
+                       a);
+    WH_HIP_CHECK(hipGetLastError());
+    return WH_OK;
+  }
+}
+
 template <typename T, int FILT>
 int modwt_launch_impl(bool inverse, const GroupLaunch& gl, const ModwtArgs& a, hipStream_t st) {
   if (gl.R == 1 && gl.NT == 512) return modwt_launch_g<T, FILT, 1, 512, 1>(inverse, gl, a, st);
+  if (inverse && gl.R == 1 && gl.NT == 256) return modwt_launch_inv256<T, FILT, 1>(gl, a, st);
   if (gl.R == 32 && gl.NT == 256) return modwt_launch_g<T, FILT, 32, 256, 1>(inverse, gl, a, st);
   return set_error(WH_ERR_UNSUPPORTED, "modwt: unsupported residue tiling");
 }
