@@ -108,12 +108,35 @@ int wh_dwt_inv(const void* y, void* x, int64_t n, int nlevels, int filter, int d
 int wh_threshold(void* coeffs, int64_t n, int nlevels, int ttype, int filter, int dtype,
                  const wh_thresh* thr, void* stream);
 
+/* The coefficients of one level gathered into the compact device array
+   `out`, in translate order: the values at the positions the reference's
+   getCoeffLevel lists (R/rplotting.R:10-46).  DWT: details of the 1-based
+   `level` (n / 2^level values from positions 2^{level-1} + m 2^level) or,
+   with coeff_type 1 and level == nlevels, the scaling coefficients
+   (n / 2^nlevels values from positions m 2^nlevels).  MODWT (time ordered):
+   D_level (coeff_type 0) or S_level (coeff_type 1), n values.  `coeffs`
+   holds a transform of length n with `nlevels` levels (0 = the maximum);
+   *count, when not NULL, receives the number of values written. */
+int wh_coeff_level(const void* coeffs, void* out, int64_t n, int nlevels, int ttype, int filter,
+                   int level, int coeff_type, int dtype, int64_t* count, void* stream);
+
+/* The label columns the reference's WSTtoDT attaches to every coefficient
+   (R/rfunctions.R:925-1004; types as label.detail.scaling, :902-912), in
+   storage order: n entries for a DWT, 2 n J for a MODWT.  level: 1-based
+   level; translate: DWT 2^{l-1} (m + 1/2) for coefficient m of level l
+   (scaling coefficients share level J's translates), MODWT i + 1 within each
+   block; type: 0 = "d", 1 = "s".  Any of the three device arrays may be
+   NULL. */
+int wh_coeff_labels(int64_t n, int nlevels, int ttype, int filter, int* level, double* translate,
+                    unsigned char* type, void* stream);
+
 /* Plan tuning for the fused MODWT: times the planner's alternatives (depth
-   of the first fused level group, segments per CU) for this length, level
-   count, filter and element type on scratch buffers of its own (allocated
-   and freed here; about (2J + 2) n elements) and keeps the fastest forward
-   and inverse plans for later calls in this process.  Results do not
-   change: every plan computes each coefficient with the same operations.
+   of the first fused level group, segments per CU, a 256-thread first
+   inverse group) for this length, level count, filter and element type on
+   scratch buffers of its own (allocated and freed here; about (2J + 2) n
+   elements) and keeps, for later calls in this process, the default plan
+   unless another is more than 2% faster.  Results do not change: every
+   plan computes each coefficient with the same operations.
    Synchronous.  No counterpart in the reference, whose GPU MODWT runs one
    kernel per level (src/haarcuda.cu:1649-1914). */
 int wh_modwt_tune(int64_t n, int nlevels, int filter, int dtype, void* stream);

@@ -641,3 +641,69 @@ int wvo_smooth(double* x, unsigned len, unsigned nlevels, int ttype, int filt, i
   ow_free(&w);
   return 0;
 }
+
+/* ---------------------------------------------------------------- levels */
+
+/* R/rplotting.R:10-46 */
+long wvo_coeff_positions(long len, unsigned nlevels, int ttype, unsigned level, int coeff_s,
+                         long* pos) {
+  if (level == 0 || level > nlevels) return 0;  /* :17-19 */
+  long start, gap;
+  if (ttype == O_DWT) {
+    if (coeff_s && level != nlevels) return 0;  /* :31-33 stopifnot */
+    start = (coeff_s ? 0 : (1L << (level - 1))) + 1; /* :35 */
+    gap = 1L << level;                               /* :36 */
+  } else {
+    start = 1; /* :39-42 */
+    gap = 1;
+  }
+  long k = 0;
+  for (long i = start; i <= len; i += gap) pos[k++] = i; /* :44 seq(start, n, by) */
+  return k;
+}
+
+/* R/rfunctions.R:902-912 (types) and :925-1004 (levels, translates) */
+int wvo_coeff_labels(long len, unsigned nlevels, int ttype, int* level, double* translate,
+                     unsigned char* type) {
+  if (nlevels == 0) return 1;
+  if (ttype == O_DWT) {
+    const long pw = 1L << nlevels;
+    for (long i = 0; i < len; ++i) {
+      type[i] = ((i + 1) % pw == 1) ? 1 : 0;   /* :909 (1:len) %% 2^nlevels == 1 */
+      level[i] = 1;                             /* :947 rep(1, len) */
+      translate[i] = 0.5 + (double)(i / 2);     /* :948 rep(seq(0.5, len/2, 1), each = 2) */
+    }
+    if (nlevels > 1) {                          /* :953 */
+      long* pos = (long*)malloc(sizeof(long) * (size_t)len);
+      if (!pos) return 2;
+      double first = 0, by = 1;
+      for (unsigned l = 2; l <= nlevels; ++l) { /* :956 */
+        long k = wvo_coeff_positions(len, nlevels, ttype, l, 0, pos);
+        first = (double)(1L << (l - 2));        /* :962 seq(2^(l-2), len/2, by = 2^(l-1)) */
+        by = (double)(1L << (l - 1));
+        for (long j = 0; j < k; ++j) {
+          level[pos[j] - 1] = (int)l;
+          translate[pos[j] - 1] = first + by * (double)j;
+        }
+      }
+      /* :967-969 the scalings take the last level and its translates */
+      long k = wvo_coeff_positions(len, nlevels, ttype, nlevels, 1, pos);
+      for (long j = 0; j < k; ++j) {
+        level[pos[j] - 1] = (int)nlevels;
+        translate[pos[j] - 1] = first + by * (double)j;
+      }
+      free(pos);
+    }
+    return 0;
+  }
+  /* MODWT, :978-985 */
+  for (unsigned l = 1; l <= nlevels; ++l)
+    for (int half = 0; half < 2; ++half)
+      for (long i = 0; i < len; ++i) {
+        const long e = 2 * len * (long)(l - 1) + half * len + i;
+        type[e] = half == 0 ? 1 : 0;   /* :911 rep(rep(c("s","d"), each = len), nlevels) */
+        level[e] = (int)l;             /* rep(1:nlevels, each = 2 len) */
+        translate[e] = (double)(i + 1); /* rep(1:len, 2 nlevels) */
+      }
+  return 0;
+}

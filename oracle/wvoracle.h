@@ -90,6 +90,20 @@ int wvo_smooth(double* x, unsigned len, unsigned nlevels, int ttype, int filt,
                int threshtype, double* thresh, int hardness,
                unsigned minlevel, unsigned maxlevel, double tol);
 
+/* getCoeffLevel (R/rplotting.R:10-46): the 1-based positions of level
+   `level`'s detail (coeff_s = 0) or scaling (coeff_s = 1) coefficients in a
+   transformed vector; returns their count (0 on an argument the R code
+   stops on).  DWT: seq(2^(level-1) * d + 1, n, by = 2^level); MODWT:
+   seq(1, n) whatever the level and type, as the R code has it. */
+long wvo_coeff_positions(long len, unsigned nlevels, int ttype, unsigned level, int coeff_s,
+                         long* pos);
+
+/* The label columns of WSTtoDT (R/rfunctions.R:925-1004) without its
+   plotting rows: CoeffType from label.detail.scaling (:902-912) as 1 = "s",
+   0 = "d"; Level; Translate.  DWT: len entries, MODWT: 2 len nlevels. */
+int wvo_coeff_labels(long len, unsigned nlevels, int ttype, int* level, double* translate,
+                     unsigned char* type);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,10 @@ def oracle():
         L.wvo_cvt.argtypes = [_DP, _U, _U, _I, _I, _I, C.c_double, _U, _U, _DP, _I,
                               C.POINTER(_I), C.POINTER(_I)]
         L.wvo_smooth.argtypes = [_DP, _U, _U, _I, _I, _I, _DP, _I, _U, _U, C.c_double]
+        L.wvo_coeff_positions.restype = C.c_long
+        L.wvo_coeff_positions.argtypes = [C.c_long, _U, _I, _U, _I, C.POINTER(C.c_long)]
+        L.wvo_coeff_labels.argtypes = [C.c_long, _U, _I, C.POINTER(_I), _DP,
+                                       C.POINTER(C.c_ubyte)]
         _o = L
     return _o
 
@@ -174,6 +178,27 @@ def smooth(x, nlevels, ttype, filt, threshtype, thresh, hardness, minlevel, maxl
     rc = oracle().wvo_smooth(_p(y), len(y), nlevels, ttype, filt, threshtype, _p(t), hardness,
                              minlevel, maxlevel, tol)
     return y, float(t[0]), rc
+
+
+def coeff_positions(n, nlevels, ttype, level, coeff_type="d"):
+    """getCoeffLevel's 1-based positions (R/rplotting.R:10-46); None where
+    the R code stops."""
+    pos = np.zeros(max(n, 1), dtype=np.int64)
+    k = oracle().wvo_coeff_positions(n, nlevels, ttype, level, 1 if coeff_type == "s" else 0,
+                                     pos.ctypes.data_as(C.POINTER(C.c_long)))
+    return pos[:k] if k > 0 else None
+
+
+def coeff_labels(n, nlevels, ttype):
+    """WSTtoDT's (level, translate, is_scaling) columns in storage order."""
+    m = n if ttype == 0 else 2 * n * nlevels
+    level = np.zeros(m, dtype=np.int32)
+    translate = np.zeros(m, dtype=np.float64)
+    kind = np.zeros(m, dtype=np.uint8)
+    rc = oracle().wvo_coeff_labels(n, nlevels, ttype, level.ctypes.data_as(C.POINTER(_I)),
+                                   _p(translate), kind.ctypes.data_as(C.POINTER(C.c_ubyte)))
+    assert rc == 0
+    return level, translate, kind
 
 
 def relerr(a, b):

@@ -66,6 +66,8 @@ SIGNATURES = {
     "RgpuSmooth": (None, [_DP, _IP, _IP, _IP, _IP, _IP, _IP, _DP, _IP, _IP, _IP, _DP]),
     "wh_debug_modwt_plan": (_I, [_I, _I64, _I, _I, _I, C.c_char_p, _I]),
     "wh_modwt_tune": (_I, [_I64, _I, _I, _I, _VP]),
+    "wh_coeff_level": (_I, [_VP, _VP, _I64, _I, _I, _I, _I, _I, _I, C.POINTER(_I64), _VP]),
+    "wh_coeff_labels": (_I, [_I64, _I, _I, _I, _VP, _VP, _VP, _VP]),
     "wh_modwt_fwd_many": (_I, [_VP, _VP, _I64, _I64, _I, _I, _I, _VP]),
     "wh_modwt_inv_many": (_I, [_VP, _VP, _I64, _I64, _I, _I, _I, C.POINTER(wh_thresh), _VP]),
     "wh_modwt_fwd_adj": (_I, [_VP, _VP, _I64, _I64, _I, _I, _I, _VP]),

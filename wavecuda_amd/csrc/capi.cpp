@@ -347,6 +347,33 @@ int wh_smooth(void* x, int64_t n, int nlevels, int ttype, int filter, int dtype,
                 tol, static_cast<hipStream_t>(stream));
 }
 
+int wh_coeff_level(const void* coeffs, void* out, int64_t n, int nlevels, int ttype, int filter,
+                   int level, int coeff_type, int dtype, int64_t* count, void* stream) {
+  if (!coeffs || !out) return set_error(WH_ERR_ARG, "null pointer");
+  if (!valid_filter(filter)) return set_error(WH_ERR_ARG, "filter must be 1 (Haar), 2 (D4), 3 (C6) or 4 (LA8)");
+  if (!valid_dtype(dtype)) return set_error(WH_ERR_ARG, "dtype must be WH_F64 or WH_F32");
+  if (ttype != kDWT && ttype != kMODWT) return set_error(WH_ERR_ARG, "ttype must be 0 (DWT) or 1 (MODWT)");
+  const int J = resolve_levels(n, nlevels, filter_length(filter));
+  if (J <= 0) return set_error(WH_ERR_ARG, "length must be a power of two and nlevels <= floor(log2(n/L))+1");
+  // getCoeffLevel's rules (R/rplotting.R:17-35)
+  if (level < 1 || level > J) return set_error(WH_ERR_ARG, "level must be an integer between 1 and nlevels");
+  if (coeff_type != 0 && coeff_type != 1) return set_error(WH_ERR_ARG, "coeff_type must be 0 (d) or 1 (s)");
+  if (ttype == kDWT && coeff_type == 1 && level != J)
+    return set_error(WH_ERR_ARG, "DWT scaling coefficients exist at level nlevels only");
+  return level_extract(coeffs, out, n, J, ttype, level, coeff_type, dtype,
+                       static_cast<hipStream_t>(stream), count);
+}
+
+int wh_coeff_labels(int64_t n, int nlevels, int ttype, int filter, int* level, double* translate,
+                    unsigned char* type, void* stream) {
+  if (!valid_filter(filter)) return set_error(WH_ERR_ARG, "filter must be 1 (Haar), 2 (D4), 3 (C6) or 4 (LA8)");
+  if (ttype != kDWT && ttype != kMODWT) return set_error(WH_ERR_ARG, "ttype must be 0 (DWT) or 1 (MODWT)");
+  const int J = resolve_levels(n, nlevels, filter_length(filter));
+  if (J <= 0) return set_error(WH_ERR_ARG, "length must be a power of two and nlevels <= floor(log2(n/L))+1");
+  if (!level && !translate && !type) return WH_OK;
+  return level_labels(n, J, ttype, level, translate, type, static_cast<hipStream_t>(stream));
+}
+
 int wh_modwt_tune(int64_t n, int nlevels, int filter, int dtype, void* stream) {
   if (!valid_filter(filter)) return set_error(WH_ERR_ARG, "filter must be 1..4");
   if (!valid_dtype(dtype)) return set_error(WH_ERR_ARG, "dtype must be WH_F64 or WH_F32");

@@ -57,6 +57,13 @@ std::string modwt_plan_string(bool inverse, int64_t n, int J, int filt, int dtyp
 // the fastest forward and inverse plans for the rest of the process
 int modwt_tune(int64_t n, int J, int filt, int dtype, hipStream_t st);
 
+// levels.hip: one level's coefficients gathered into out (count written to
+// *count_out), and the per-coefficient label columns
+int level_extract(const void* c, void* out, int64_t n, int J, int ttype, int level, int coeff_s,
+                  int dtype, hipStream_t st, int64_t* count_out);
+int level_labels(int64_t n, int J, int ttype, int* level, double* translate, unsigned char* type,
+                 hipStream_t st);
+
 int dwt_forward(const void* x, void* y, int64_t n, int J, int filt, int dtype, hipStream_t st);
 int dwt_inverse(const void* y, void* x, int64_t n, int J, int filt, int dtype,
                 const wh_thresh* thr, hipStream_t st);

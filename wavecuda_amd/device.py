@@ -142,7 +142,8 @@ def dwt_inv(y: torch.Tensor, nlevels: int = 0, filter="LA8", thresh=None,
 
 def modwt_tune(n: int, nlevels: int, filter="LA8", dtype=torch.float64) -> None:
     """Time the fused MODWT's plan alternatives for this shape and keep the
-    fastest for later calls (synchronous)."""
+    default plan for later calls unless another is more than 2% faster
+    (synchronous; results do not depend on the plan)."""
     f = filter_code(filter)
     W.check(W.lib().wh_modwt_tune(int(n), resolve_levels(n, nlevels, f), f, _DT[dtype],
                                   _stream()), "wh_modwt_tune")
@@ -154,6 +155,46 @@ def modwt_plan(inverse: bool, n: int, nlevels: int, filter="LA8", dtype=torch.fl
                                     _DT[dtype], buf, 4096)
     W.check(rc, "wh_debug_modwt_plan")
     return buf.value.decode()
+
+
+def coeff_level(coeffs: torch.Tensor, n: int, nlevels: int, ttype: int, filter, level: int,
+                coeff_type: str = "d") -> torch.Tensor:
+    """One level's coefficients of a transformed vector (DWT n / MODWT 2nJ
+    values) as a new compact tensor in translate order: the values at the
+    positions getCoeffLevel (R/rplotting.R:10-46) lists for a DWT, the
+    level's D or S block for a time-ordered MODWT.  DWT scaling coefficients
+    ("s") exist at the deepest level only."""
+    _check_dev(coeffs)
+    f = filter_code(filter)
+    J = resolve_levels(n, nlevels, f)
+    if coeff_type not in ("d", "s"):
+        raise ValueError('coeff_type must be "d" or "s"')
+    if not 1 <= level <= J:
+        raise ValueError(f"level must be between 1 and {J}")
+    s = coeff_type == "s"
+    count = (n >> (J if s else level)) if int(ttype) == 0 else n
+    out = torch.empty(max(count, 1), dtype=coeffs.dtype, device=coeffs.device)
+    got = C.c_int64(0)
+    W.check(W.lib().wh_coeff_level(coeffs.data_ptr(), out.data_ptr(), n, J, int(ttype), f,
+                                   int(level), 1 if s else 0, _dtype(coeffs), C.byref(got),
+                                   _stream()), "wh_coeff_level")
+    return out[: got.value]
+
+
+def coeff_labels(n: int, nlevels: int, ttype: int, filter="Haar"):
+    """WSTtoDT's label columns (R/rfunctions.R:925-1004) for every
+    coefficient in storage order, computed on the device: (level int32,
+    translate float64, is_scaling uint8) tensors of length n (DWT) or 2nJ
+    (MODWT)."""
+    f = filter_code(filter)
+    J = resolve_levels(n, nlevels, f)
+    m = n if int(ttype) == 0 else 2 * n * J
+    level = torch.empty(m, dtype=torch.int32, device="cuda")
+    translate = torch.empty(m, dtype=torch.float64, device="cuda")
+    kind = torch.empty(m, dtype=torch.uint8, device="cuda")
+    W.check(W.lib().wh_coeff_labels(n, J, int(ttype), f, level.data_ptr(), translate.data_ptr(),
+                                    kind.data_ptr(), _stream()), "wh_coeff_labels")
+    return level, translate, kind
 
 
 def threshold_(coeffs: torch.Tensor, n: int, nlevels: int, ttype: int, filter, value: float,

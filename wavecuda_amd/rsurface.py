@@ -235,3 +235,76 @@ def GPUSmooth(xin, nlevels, transform_type, filter, thresh_type, thresh=None, ha
                        _ip(a["filtlen"]), _ip(a["threshtype"]), _dp(t), _ip(a["hardness"]),
                        _ip(a["min_level"]), _ip(a["max_level"]), _dp(tl))
     return x
+
+
+# ---- coefficients by level: getCoeffLevel, label.detail.scaling, WSTtoDT --
+
+def getCoeffLevel(xw: WST, level, coeffType="d"):
+    """getCoeffLevel (R/rplotting.R:10-46): the 1-based positions of level
+    `level`'s detail ("d") or scaling ("s") coefficients in xw.w.  As in the
+    R code, a MODWT gives 1..len whatever the level and type; the device
+    gather `device.coeff_level` returns the level's own D or S block."""
+    if not isinstance(xw, WST):
+        raise ValueError("WST type required")
+    if level <= 0 or level > xw.nlevels or level != int(level):
+        raise ValueError(f"Integer level expected, between 1 and {xw.nlevels}")
+    if coeffType not in ("d", "s"):
+        raise ValueError('coeffType must be "d" or "s"')
+    d = coeffType == "d"
+    n = xw.len
+    if xw.ttype == "DWT":
+        if not d and level != xw.nlevels:
+            raise ValueError("scaling coefficients exist at level nlevels only")
+        start, gap = 2 ** (int(level) - 1) * d + 1, 2 ** int(level)
+    else:
+        start, gap = 1, 1
+    return np.arange(start, n + 1, gap, dtype=np.int64)
+
+
+def _labels(xwav: WST):
+    """(level, translate, is_scaling) of every coefficient, from the device."""
+    from . import device as D
+
+    lev, tr, kind = D.coeff_labels(xwav.len, xwav.nlevels, _TTYPE[xwav.ttype], xwav.filt)
+    return lev.cpu().numpy(), tr.cpu().numpy(), kind.cpu().numpy()
+
+
+def label_detail_scaling(Xwav: WST):
+    """label.detail.scaling (R/rfunctions.R:902-912): "s" or "d" for each
+    coefficient in storage order (len of them for a DWT, 2 len nlevels for a
+    MODWT), computed on the device."""
+    kind = _labels(Xwav)[2]
+    return np.where(kind == 1, "s", "d")
+
+
+def WSTtoDT(Xwav: WST, scaling=True, forPlotting=False):
+    """WSTtoDT (R/rfunctions.R:925-1004) as a pandas DataFrame with the
+    columns W, CoeffType, Level, Translate (and minmax when forPlotting),
+    one row per coefficient in storage order; the label columns come from
+    the device.  forPlotting drops the scaling rows and appends, for every
+    level, two rows of Translate 0 holding -max|W| and +max|W| of that
+    level's details (minmax 1 and 2).  A MODWT's rows are its 2 len nlevels
+    coefficients (R pads w to 2 len log2(len))."""
+    import pandas as pd
+
+    lev, tr, kind = _labels(Xwav)
+    w = np.asarray(Xwav.w, dtype=np.float64)[: lev.size]
+    if forPlotting:
+        scaling = False
+    df = pd.DataFrame({"W": w, "CoeffType": np.where(kind == 1, "s", "d"),
+                       "Level": lev.astype(np.int64), "Translate": tr,
+                       "minmax": np.zeros(lev.size, dtype=np.int64)})
+    if forPlotting:
+        pads = []
+        for level in range(1, Xwav.nlevels + 1):
+            sel = (df["CoeffType"] == "d") & (df["Level"] == level)
+            m = float(np.abs(df.loc[sel, "W"]).max()) if sel.any() else np.nan
+            for mm, sign in ((1, -1.0), (2, 1.0)):
+                pads.append({"W": sign * m, "CoeffType": "d", "Level": level,
+                             "Translate": 0.0, "minmax": mm})
+        df = pd.concat([df, pd.DataFrame(pads)], ignore_index=True)
+    else:
+        df = df.drop(columns="minmax")
+    if not scaling:
+        df = df[df["CoeffType"] == "d"].reset_index(drop=True)
+    return df
