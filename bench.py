@@ -269,6 +269,35 @@ def small_section(steps):
     return {"workload": "D4 DWT fwd+inv, n=1024, 9 levels, FP64", "us_per_fwd_inv": round(ms * 1e3, 2)}
 
 
+def levels_section(steps):
+    """Coefficient layouts at the headline size (LA8, n = 2^24, J = 10, FP64):
+    time order <-> packet order (every value read and written once: 16 B per
+    value) and WSTtoDT's label columns (13 B written per coefficient)."""
+    import torch
+
+    from wavecuda_amd import device as D
+
+    n, J = 1 << 24, 10
+    m = 2 * n * J
+    g = torch.Generator(device="cuda").manual_seed(1)
+    to = torch.randn(m, dtype=torch.float64, device="cuda", generator=g)
+    po = torch.empty_like(to)
+    back = torch.empty_like(to)
+    t_to = _time_ms(lambda: D.modwt_convert(to, n, J, "LA8", True, out=po), steps)
+    t_po = _time_ms(lambda: D.modwt_convert(po, n, J, "LA8", False, out=back), steps)
+    ok = bool(torch.equal(back, to))
+    del to, po, back
+    t_lab = _time_ms(lambda: D.coeff_labels(n, J, 1, "LA8"), steps)
+    b = 16.0 * m
+    return {"workload": "LA8 MODWT J=10 coefficients, n=2^24, FP64 (2nJ = 335.5 M values)",
+            "to_po_ms": round(t_to, 4), "to_po_GBs": round(b / (t_to * 1e-3) / 1e9, 1),
+            "po_to_ms": round(t_po, 4), "po_to_GBs": round(b / (t_po * 1e-3) / 1e9, 1),
+            "hbm_frac": round(b / (max(t_to, t_po) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "round_trip_exact": ok,
+            "labels_ms": round(t_lab, 4),
+            "labels_GBs": round(13.0 * m / (t_lab * 1e-3) / 1e9, 1)}
+
+
 def smooth_section():
     """Single-signal smoothing, n = 2^18 doppler + noise, MODWT J = 10, Haar
     and LA8, soft and hard, universal and cross-validated thresholds, SNR 5
@@ -545,6 +574,7 @@ def main():
     if not args.no_extra:
         out["sweep"] = sweep_section(max(5, args.steps // 5))
         out["small"] = small_section(max(5, args.steps // 5))
+        out["layouts"] = levels_section(max(5, args.steps // 10))
         out["smooth"] = smooth_section()
         out["differentiable"] = diff_section()
         out["host_c_abi"] = host_section()

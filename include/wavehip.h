@@ -74,7 +74,9 @@ int wh_resolve_levels(int64_t n, int nlevels, int filter);
 
 /* ---- reference .C entry points (host pointers, synchronous) ----------- */
 
-/* reference src/rfunctions.cuh:22 / src/rfunctions.cu:21-57 */
+/* reference src/rfunctions.cuh:22 / src/rfunctions.cu:21-57; ttype 2 is
+   the packet-ordered MODWT of the reference's CPU dispatch
+   (src/transform.cpp:39-55), here for all four filters. */
 void RgpuTransform(double* x, double* xmod, int* len, int* sense, int* nlevels, int* ttype,
                    int* filter, int* filterlen);
 /* reference src/rfunctions.cuh:34 / src/rfunctions.cu:102-136; all four
@@ -107,6 +109,18 @@ int wh_dwt_inv(const void* y, void* x, int64_t n, int nlevels, int filter, int d
    2 n J values).  reference src/thresh.cpp:15-83. */
 int wh_threshold(void* coeffs, int64_t n, int nlevels, int ttype, int filter, int dtype,
                  const wh_thresh* thr, void* stream);
+
+/* Every level of a MODWT coefficient vector (2 n J values) between time
+   order (ttype 1) and packet order (reference src/wavutils.cpp:532-600,
+   convert_modwt): to_po = 1 reads time order and writes packet order,
+   to_po = 0 the reverse.  Level l (0-based) of the packet order holds
+   2^{l+1} packets of n / 2^l values, (s, d) pairs, packet `shift` carrying
+   the time positions rev(shift) + 2^{l+1} k (rev: the l + 1 low bits
+   reversed).  Out of place (in != out); a permutation, so the result is
+   exact.  The packet-ordered transforms are the time-ordered ones plus
+   this conversion (RgpuTransform with ttype 2). */
+int wh_modwt_convert(const void* in, void* out, int64_t n, int nlevels, int filter, int to_po,
+                     int dtype, void* stream);
 
 /* The coefficients of one level gathered into the compact device array
    `out`, in translate order: the values at the positions the reference's

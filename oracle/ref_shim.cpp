@@ -99,4 +99,15 @@ unsigned ref_check_len_levels(unsigned len, unsigned nlevels, unsigned filterlen
   return check_len_levels(len, nlevels, filterlength);
 }
 
+// convert_modwt (src/wavutils.cpp:532-567) on caller buffers: every level's
+// 2 len block of xin converted from intype (1 = time ordered, 2 = packet
+// ordered) to the other order, into xout
+int ref_convert_modwt(double* xin, double* xout, int len, int nlevels, int intype) {
+  for (int l = 0; l < nlevels; ++l)
+    if (convert_modwt_level(xin + 2 * (size_t)len * l, xout + 2 * (size_t)len * l, (uint)l,
+                            (short)intype, (uint)len))
+      return 1;
+  return 0;
+}
+
 }  // extern "C"

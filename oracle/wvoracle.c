@@ -707,3 +707,56 @@ int wvo_coeff_labels(long len, unsigned nlevels, int ttype, int* level, double* 
       }
   return 0;
 }
+
+/* ------------------------------------------------------- packet ordering */
+
+/* reverse the `bits` low bits of v (src/utils.cpp:254-274) */
+static unsigned o_reverse_bits(unsigned v, unsigned bits) {
+  unsigned r = 0;
+  for (unsigned b = 0; b < bits; ++b) r |= ((v >> b) & 1u) << (bits - 1 - b);
+  return r;
+}
+
+/* src/wavutils.cpp:532-600 */
+int wvo_modwt_convert(const double* in, double* out, unsigned len, unsigned nlevels, int intype) {
+  if (intype != 1 && intype != 2) return 1;
+  for (unsigned l = 0; l < nlevels; ++l) {
+    const double* xin = in + 2 * (size_t)len * l;
+    double* xout = out + 2 * (size_t)len * l;
+    const unsigned skip = 1u << l, lenos = len / skip, npackets = skip << 1;
+    for (unsigned shift = 0; shift < npackets; ++shift) {
+      const unsigned to = o_reverse_bits(shift, l + 1);
+      for (unsigned k = 0; 2 * k < lenos; ++k) {
+        const size_t po = (size_t)shift * lenos + 2 * k, t = to + (size_t)2 * skip * k;
+        if (intype == 2) { /* :586-589 */
+          xout[t] = xin[po];
+          xout[len + t] = xin[po + 1];
+        } else {           /* :590-593 */
+          xout[po] = xin[t];
+          xout[po + 1] = xin[len + t];
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+int wvo_modwt_po(double* x, double* xmod, unsigned len, int sense, unsigned nlevels, int filt) {
+  int L = wvo_filter_length(filt);
+  if (L == 0) return 2;
+  unsigned J = wvo_check_len_levels(len, nlevels, (unsigned)L);
+  if (J == 0) return 3;
+  const size_t m = 2 * (size_t)len * J;
+  double* tmp = (double*)malloc(sizeof(double) * m);
+  if (!tmp) return 5;
+  int rc;
+  if (sense == O_FWD) {
+    rc = wvo_modwt(x, tmp, len, O_FWD, J, filt);
+    if (!rc) rc = wvo_modwt_convert(tmp, xmod, len, J, 1);
+  } else {
+    rc = wvo_modwt_convert(xmod, tmp, len, J, 2);
+    if (!rc) rc = wvo_modwt(x, tmp, len, O_BWD, J, filt);
+  }
+  free(tmp);
+  return rc;
+}

@@ -104,6 +104,20 @@ long wvo_coeff_positions(long len, unsigned nlevels, int ttype, unsigned level, 
 int wvo_coeff_labels(long len, unsigned nlevels, int ttype, int* level, double* translate,
                      unsigned char* type);
 
+/* convert_modwt / convert_modwt_level (src/wavutils.cpp:532-600): every
+   level's 2 len block of `in` from intype (1 = time ordered, 2 = packet
+   ordered) to the other order, into `out`.  Packet `shift` of level l
+   (0-based; 2^{l+1} packets of len / 2^l values, (s, d) pairs) holds the
+   time positions shift_to + 2^{l+1} k, shift_to = shift with its l + 1 low
+   bits reversed (:579-583). */
+int wvo_modwt_convert(const double* in, double* out, unsigned len, unsigned nlevels, int intype);
+
+/* The packet-ordered MODWT of src/transform.cpp:39-55 (the reference has
+   Haar and D4; the same construction serves C6 and LA8): the time-ordered
+   transform and the conversion (forward), or the conversion and the
+   time-ordered inverse (BWD; xmod is left in packet order). */
+int wvo_modwt_po(double* x, double* xmod, unsigned len, int sense, unsigned nlevels, int filt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -176,6 +176,36 @@ def main():
         put(k + "_y", y)
         put(k + "_t", t)
 
+    # ---- packet-ordered MODWT (src/transform.cpp:39-55: Haar, D4) and the
+    # PO <-> TO conversion (src/wavutils.cpp:532-600), from a stream of
+    # their own so the vectors above do not change
+    rpo = np.random.Generator(np.random.PCG64(4242))
+    for f in (1, 2):
+        for n in (16, 64, 256, 1024):
+            for nl in (1, 2, 0):
+                J = R.ref_check_len_levels(n, nl, FLEN[f])
+                if J == 0 or (n == 1024 and nl == 0):
+                    continue
+                x = rpo.standard_normal(n)
+                xm = np.zeros(2 * n * J)
+                assert R.ref_transform(p(x.copy()), p(xm), n, 1, J, 2, f, FLEN[f]) == 0
+                xr = np.zeros(n)
+                assert R.ref_transform(p(xr), p(xm.copy()), n, 0, J, 2, f, FLEN[f]) == 0
+                k = f"modwtpo_f{f}_n{n}_l{J}"
+                put(k + "_x", x)
+                put(k + "_fwd", xm)
+                put(k + "_inv", xr)
+    for n, J in ((8, 3), (64, 6), (256, 4), (1024, 10), (4096, 5)):
+        v = rpo.standard_normal(2 * n * J)
+        po = np.zeros_like(v)
+        assert R.ref_convert_modwt(p(v), p(po), n, J, 1) == 0
+        to = np.zeros_like(v)
+        assert R.ref_convert_modwt(p(v), p(to), n, J, 2) == 0
+        k = f"convert_n{n}_l{J}"
+        put(k + "_in", v)
+        put(k + "_to2po", po)
+        put(k + "_po2to", to)
+
     path = HERE / "reference_vectors.npz"
     np.savez_compressed(path, **out)
     print(f"wrote {len(out)} arrays to {path} ({path.stat().st_size} bytes)")

@@ -61,6 +61,8 @@ def oracle():
         L.wvo_cvt.argtypes = [_DP, _U, _U, _I, _I, _I, C.c_double, _U, _U, _DP, _I,
                               C.POINTER(_I), C.POINTER(_I)]
         L.wvo_smooth.argtypes = [_DP, _U, _U, _I, _I, _I, _DP, _I, _U, _U, C.c_double]
+        L.wvo_modwt_convert.argtypes = [_DP, _DP, _U, _U, _I]
+        L.wvo_modwt_po.argtypes = [_DP, _DP, _U, _I, _U, _I]
         L.wvo_coeff_positions.restype = C.c_long
         L.wvo_coeff_positions.argtypes = [C.c_long, _U, _I, _U, _I, C.POINTER(C.c_long)]
         L.wvo_coeff_labels.argtypes = [C.c_long, _U, _I, C.POINTER(_I), _DP,
@@ -90,6 +92,7 @@ def ref():
         L.ref_interp_mse.argtypes = [_DP, _DP, _DP, _I]
         L.ref_check_len_levels.restype = _U
         L.ref_check_len_levels.argtypes = [_U, _U, _U]
+        L.ref_convert_modwt.argtypes = [_DP, _DP, _I, _I, _I]
         _r = L
     return _r
 
@@ -178,6 +181,31 @@ def smooth(x, nlevels, ttype, filt, threshtype, thresh, hardness, minlevel, maxl
     rc = oracle().wvo_smooth(_p(y), len(y), nlevels, ttype, filt, threshtype, _p(t), hardness,
                              minlevel, maxlevel, tol)
     return y, float(t[0]), rc
+
+
+def modwt_convert(xmod, n, nlevels, intype):
+    """convert_modwt: intype 1 = time ordered in, 2 = packet ordered in."""
+    a = np.ascontiguousarray(xmod, dtype=np.float64)
+    out = np.zeros_like(a)
+    assert oracle().wvo_modwt_convert(_p(a), _p(out), n, nlevels, intype) == 0
+    return out
+
+
+def modwt_po_fwd(x, nlevels, filt):
+    n = x.size
+    J = levels(n, nlevels, filt)
+    xm = np.zeros(2 * n * J)
+    assert oracle().wvo_modwt_po(_p(np.ascontiguousarray(x, dtype=np.float64).copy()), _p(xm),
+                                 n, 1, J, filt) == 0
+    return xm
+
+
+def modwt_po_inv(xmod, n, nlevels, filt):
+    J = levels(n, nlevels, filt)
+    x = np.zeros(n)
+    assert oracle().wvo_modwt_po(_p(x), _p(np.ascontiguousarray(xmod, dtype=np.float64).copy()),
+                                 n, 0, J, filt) == 0
+    return x
 
 
 def coeff_positions(n, nlevels, ttype, level, coeff_type="d"):

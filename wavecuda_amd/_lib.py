@@ -11,11 +11,12 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "libwavehip.so"
 
 WH_OK = 0
+WH_ERR_ARG, WH_ERR_HIP, WH_ERR_NOMEM, WH_ERR_UNSUPPORTED, WH_ERR_NODEVICE = 1, 2, 3, 4, 5
 WH_F64 = 0
 WH_F32 = 1
 
 FWD, BWD = 1, 0
-DWT, MODWT = 0, 1
+DWT, MODWT, MODWT_PO = 0, 1, 2
 HAAR, D4, C6, LA8 = 1, 2, 3, 4
 HARD, SOFT = 0, 1
 MANUAL, UNIV, CV = 0, 1, 2
@@ -67,6 +68,7 @@ SIGNATURES = {
     "wh_debug_modwt_plan": (_I, [_I, _I64, _I, _I, _I, C.c_char_p, _I]),
     "wh_modwt_tune": (_I, [_I64, _I, _I, _I, _VP]),
     "wh_coeff_level": (_I, [_VP, _VP, _I64, _I, _I, _I, _I, _I, _I, C.POINTER(_I64), _VP]),
+    "wh_modwt_convert": (_I, [_VP, _VP, _I64, _I, _I, _I, _I, _VP]),
     "wh_coeff_labels": (_I, [_I64, _I, _I, _I, _VP, _VP, _VP, _VP]),
     "wh_modwt_fwd_many": (_I, [_VP, _VP, _I64, _I64, _I, _I, _I, _VP]),
     "wh_modwt_inv_many": (_I, [_VP, _VP, _I64, _I64, _I, _I, _I, C.POINTER(wh_thresh), _VP]),

@@ -364,6 +364,15 @@ int wh_coeff_level(const void* coeffs, void* out, int64_t n, int nlevels, int tt
                        static_cast<hipStream_t>(stream), count);
 }
 
+int wh_modwt_convert(const void* in, void* out, int64_t n, int nlevels, int filter, int to_po,
+                     int dtype, void* stream) {
+  int J, rc = check_common(in, out, n, nlevels, filter, dtype, &J);
+  if (rc) return rc;
+  if (in == out) return set_error(WH_ERR_ARG, "modwt convert: input and output must differ");
+  if (to_po != 0 && to_po != 1) return set_error(WH_ERR_ARG, "to_po must be 0 or 1");
+  return modwt_order(in, out, n, J, to_po == 1, dtype, static_cast<hipStream_t>(stream));
+}
+
 int wh_coeff_labels(int64_t n, int nlevels, int ttype, int filter, int* level, double* translate,
                     unsigned char* type, void* stream) {
   if (!valid_filter(filter)) return set_error(WH_ERR_ARG, "filter must be 1 (Haar), 2 (D4), 3 (C6) or 4 (LA8)");
@@ -439,20 +448,30 @@ void RgpuTransform(double* x, double* xmod, int* len, int* sense, int* nlevels, 
       return report("transform", set_error(WH_ERR_HIP, "copy out"));
     return;
   }
-  if (*ttype == kMODWT) {
+  if (*ttype == kMODWT || *ttype == kMODWT_PO) {
+    // packet order (ttype 2, the reference's CPU dispatch of
+    // src/transform.cpp:39-55) is the time-ordered transform plus the
+    // conversion of src/wavutils.cpp:532-600 -- bit-identical to the
+    // reference's own packet-ordered forward
     if (!xmod) return report("transform", set_error(WH_ERR_ARG, "null xmod"));
+    const bool po = *ttype == kMODWT_PO;
     const size_t nm = size_t(2) * size_t(n) * size_t(J);
-    DevBuf a, b;
-    if ((rc = a.alloc(n * 8)) || (rc = b.alloc(nm * 8))) return report("transform", rc);
+    DevBuf a, b, c;
+    if ((rc = a.alloc(n * 8)) || (rc = b.alloc(nm * 8)) || (po && (rc = c.alloc(nm * 8))))
+      return report("transform", rc);
     if (*sense == kFWD) {
       if (hipMemcpy(a.p, x, n * 8, hipMemcpyHostToDevice) != hipSuccess)
         return report("transform", set_error(WH_ERR_HIP, "copy in"));
       if ((rc = wh_modwt_fwd(a.p, b.p, n, J, *filter, WH_F64, nullptr))) return report("transform", rc);
-      if (hipMemcpy(xmod, b.p, nm * 8, hipMemcpyDeviceToHost) != hipSuccess)
+      if (po && (rc = wh_modwt_convert(b.p, c.p, n, J, *filter, 1, WH_F64, nullptr)))
+        return report("transform", rc);
+      if (hipMemcpy(xmod, po ? c.p : b.p, nm * 8, hipMemcpyDeviceToHost) != hipSuccess)
         return report("transform", set_error(WH_ERR_HIP, "copy out"));
     } else {
-      if (hipMemcpy(b.p, xmod, nm * 8, hipMemcpyHostToDevice) != hipSuccess)
+      if (hipMemcpy(po ? c.p : b.p, xmod, nm * 8, hipMemcpyHostToDevice) != hipSuccess)
         return report("transform", set_error(WH_ERR_HIP, "copy in"));
+      if (po && (rc = wh_modwt_convert(c.p, b.p, n, J, *filter, 0, WH_F64, nullptr)))
+        return report("transform", rc);
       if ((rc = wh_modwt_inv(b.p, a.p, n, J, *filter, WH_F64, nullptr, nullptr)))
         return report("transform", rc);
       if (hipMemcpy(x, a.p, n * 8, hipMemcpyDeviceToHost) != hipSuccess)
@@ -460,7 +479,7 @@ void RgpuTransform(double* x, double* xmod, int* len, int* sense, int* nlevels, 
     }
     return;
   }
-  report("transform", set_error(WH_ERR_ARG, "ttype must be 0 (DWT) or 1 (MODWT)"));
+  report("transform", set_error(WH_ERR_ARG, "ttype must be 0 (DWT), 1 (MODWT) or 2 (packet-ordered MODWT)"));
 }
 
 void RgpuSmooth(double* x, int* len, int* nlevels, int* ttype, int* filter, int* filterlen,

@@ -13,7 +13,7 @@
 namespace wh {
 
 enum { kFWD = 1, kBWD = 0 };
-enum { kDWT = 0, kMODWT = 1 };
+enum { kDWT = 0, kMODWT = 1, kMODWT_PO = 2 };  // kMODWT: time ordered
 enum { kHAAR = 1, kD4 = 2, kC6 = 3, kLA8 = 4 };
 enum { kHARD = 0, kSOFT = 1 };
 

@@ -64,6 +64,11 @@ int level_extract(const void* c, void* out, int64_t n, int J, int ttype, int lev
 int level_labels(int64_t n, int J, int ttype, int* level, double* translate, unsigned char* type,
                  hipStream_t st);
 
+// modwt_order.hip: every level of a 2nJ MODWT vector between time order and
+// packet order (to_po: time ordered in), out of place
+int modwt_order(const void* in, void* out, int64_t n, int J, bool to_po, int dtype,
+                hipStream_t st);
+
 int dwt_forward(const void* x, void* y, int64_t n, int J, int filt, int dtype, hipStream_t st);
 int dwt_inverse(const void* y, void* x, int64_t n, int J, int filt, int dtype,
                 const wh_thresh* thr, hipStream_t st);

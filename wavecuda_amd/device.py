@@ -157,6 +157,35 @@ def modwt_plan(inverse: bool, n: int, nlevels: int, filter="LA8", dtype=torch.fl
     return buf.value.decode()
 
 
+def modwt_convert(xmod: torch.Tensor, n: int, nlevels: int, filter="LA8", to_po: bool = True,
+                  out: torch.Tensor | None = None):
+    """A MODWT coefficient vector (2nJ values) from time order to packet
+    order (to_po) or back (convert_modwt, src/wavutils.cpp:532-600)."""
+    _check_dev(xmod)
+    f = filter_code(filter)
+    J = resolve_levels(n, nlevels, f)
+    if xmod.numel() < 2 * n * J:
+        raise ValueError(f"xmod must hold 2 n J = {2 * n * J} values")
+    if out is None:
+        out = torch.empty(2 * n * J, dtype=xmod.dtype, device=xmod.device)
+    _check_dev(out)
+    W.check(W.lib().wh_modwt_convert(xmod.data_ptr(), out.data_ptr(), n, J, f,
+                                     1 if to_po else 0, _dtype(xmod), _stream()),
+            "wh_modwt_convert")
+    return out
+
+
+def modwt_fwd_po(x: torch.Tensor, nlevels: int = 0, filter="LA8"):
+    """Packet-ordered MODWT forward (src/transform.cpp:39-55)."""
+    return modwt_convert(modwt_fwd(x, nlevels, filter), x.numel(), nlevels, filter, True)
+
+
+def modwt_inv_po(xmod_po: torch.Tensor, n: int, nlevels: int = 0, filter="LA8", thresh=None):
+    """Inverse of a packet-ordered MODWT vector."""
+    return modwt_inv(modwt_convert(xmod_po, n, nlevels, filter, False), n, nlevels, filter,
+                     thresh)
+
+
 def coeff_level(coeffs: torch.Tensor, n: int, nlevels: int, ttype: int, filter, level: int,
                 coeff_type: str = "d") -> torch.Tensor:
     """One level's coefficients of a transformed vector (DWT n / MODWT 2nJ
