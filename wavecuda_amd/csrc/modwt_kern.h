@@ -20,11 +20,12 @@
 // (L-1)(2^G - 1) positions (rounded up to M) whose outputs are computed but
 // not stored -- the only redundant work.
 //
-// Each thread owns two elements.  For R = 1 they are the outputs q and
-// q + s of one residue class of the level's stride s, whose L-tap windows
-// overlap, so a level reads L + 1 values per thread instead of 2L (for
-// s = 1 the two are adjacent and read as 16-byte pairs); for R > 1 they are
-// two adjacent residues and every access is a 16-byte vector.  Ring slots
+// Each thread owns two elements (four in the first inverse group's levels,
+// see InvLinCtx).  For R = 1 they are the outputs q and q + s of one
+// residue class of the level's stride s, whose L-tap windows overlap, so a
+// level reads L + 1 values per thread instead of 2L (for s = 1 the two are
+// adjacent and read as 16-byte pairs); for R > 1 they are two adjacent
+// residues and every access is a 16-byte vector.  Ring slots
 // are kept even so a pair never straddles the ring's wrap; the inverse
 // interleaves S and D per slot (R = 1) so one 16-byte read returns both.
 //
