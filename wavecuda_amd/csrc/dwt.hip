@@ -69,10 +69,12 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
   constexpr int L = Filter<FILT>::L;
   constexpr int HMAX = halo_fwd(kGmax, L);
   constexpr int SZA = kTile + HMAX;
-  constexpr int SZB = kTile / 2 + halo_fwd(kGmax - 1, L);
+  // outputs per thread at level 1 (the widest level); each level's s is
+  // computed into registers and written back over its input after a
+  // barrier, so one buffer serves every level (four workgroups per CU)
+  constexpr int kPer = (kTile / 2 + halo_fwd(kGmax - 1, L) + kNT - 1) / kNT;
   using V = typename Vec2<T>::type;
   __shared__ __align__(16) T bufA[SZA + 2];
-  __shared__ __align__(16) T bufB[SZB + 2];
   __shared__ __align__(16) T dbuf[kTile];
 
   const int tid = threadIdx.x;
@@ -88,29 +90,36 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
   __syncthreads();
 
   T* cur = bufA;
-  T* nxt = bufB;
   for (int l = 1; l <= g; ++l) {
     const int own = kTile >> l;
     const int Kl = own + halo_fwd(g - l, L);
     const int doff = kTile - (kTile >> (l - 1));
-    for (int i = tid; i < Kl; i += kNT) {
-      // taps 2i .. 2i+L-1 as L/2 aligned 16-byte pairs (no bank conflicts)
-      T v[L];
+    T sreg[kPer];
 #pragma unroll
-      for (int k = 0; k < L / 2; ++k) {
-        const V p = *reinterpret_cast<const V*>(cur + 2 * i + 2 * k);
-        v[2 * k] = p.x;
-        v[2 * k + 1] = p.y;
+    for (int k = 0; k < kPer; ++k) {
+      const int i = tid + k * kNT;
+      if (i < Kl) {
+        // taps 2i .. 2i+L-1 as L/2 aligned 16-byte pairs (no bank conflicts)
+        T v[L];
+#pragma unroll
+        for (int kk = 0; kk < L / 2; ++kk) {
+          const V p = *reinterpret_cast<const V*>(cur + 2 * i + 2 * kk);
+          v[2 * kk] = p.x;
+          v[2 * kk + 1] = p.y;
+        }
+        T sv, d;
+        fwd_pair<T, FILT>(v, sv, d);
+        sreg[k] = sv;
+        if (i < own) dbuf[doff + i] = d;
       }
-      T s, d;
-      fwd_pair<T, FILT>(v, s, d);
-      nxt[i] = s;
-      if (i < own) dbuf[doff + i] = d;
+    }
+    __syncthreads();  // every tap of this level read before s_l overwrites it
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = tid + k * kNT;
+      if (i < Kl) cur[i] = sreg[k];
     }
     __syncthreads();
-    T* t = cur;
-    cur = nxt;
-    nxt = t;
   }
   // cur holds s_{j0+g}; own part is cur[0 .. kTile >> g)
   const int j0 = a.j0;
@@ -153,11 +162,15 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
   // below it (the lowest detail any of its levels reads)
   constexpr int HY = (L - 2) << kGmax;
   using V = typename Vec2<T>::type;
-  __shared__ __align__(16) T pool[(kTile + PADS) + (kTile / 2 + PADS)];
+  // the pool holds the staged y span, then s of the current level: each
+  // level's outputs go through registers (at most kPairs pairs a thread)
+  // and back over its input after a barrier, so one buffer serves every
+  // level (four workgroups per CU)
+  constexpr int kPool = kTile + (HY > PADS ? HY : PADS);
+  constexpr int kPairs = (kTile / 2 + kNT - 1) / kNT;
+  __shared__ __align__(16) T pool[kPool];
   __shared__ T dbuf[kTile + PADS * kGmax];
-  static_assert(kTile + HY <= (kTile + PADS) + (kTile / 2 + PADS), "y staging fits the pool");
-  T* bufX = pool;
-  T* bufY = pool + kTile + PADS;
+  T* bufS = pool;
 
   const int tid = threadIdx.x;
   const int g = a.g, j0 = a.j0;
@@ -212,7 +225,7 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
     const int cnt = own + eg;
     const int64_t Ng = N >> g;
     const int64_t Bg = base >> g;
-    T* top = (g & 1) ? bufY : bufX;  // level g reads the buffer level g+1 would write
+    T* top = bufS;
     for (int i = tid; i < cnt; i += kNT) {
       int64_t m = (Bg - eg + i) & (Ng - 1);
       top[i] = src[m << a.src_shift];
@@ -222,8 +235,7 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
   // levels g..1: level l reads s_l, d_l (start B_l - e_l) and writes
   // s_{l-1} over local [0, own_{l-1} + e_{l-1}) (start B_{l-1} - e_{l-1})
   for (int l = g; l >= 1; --l) {
-    const T* cur = (l & 1) ? bufY : bufX;
-    T* out = (l & 1) ? bufX : bufY;
+    T* cur = bufS;
     int dof = 0;
     for (int ll = 1; ll < l; ++ll) dof += (kTile >> ll) + PADS;
     const T* dl = dbuf + dof;
@@ -231,23 +243,32 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
     const int npairs = ((kTile >> (l - 1)) + elm) / 2;
     // pair p has global index m = B_l - e_{l-1}/2 + p, local e_l - e_{l-1}/2 + p
     const int off = evn(l) - elm / 2;
-    for (int p = tid; p < npairs; p += kNT) {
-      T sv[L / 2], dv[L / 2];
+    V res[kPairs];
 #pragma unroll
-      for (int q = 0; q < L / 2; ++q) {
-        sv[q] = cur[off + p - q];
-        dv[q] = dl[off + p - q];
+    for (int k = 0; k < kPairs; ++k) {
+      const int p = tid + k * kNT;
+      if (p < npairs) {
+        T sv[L / 2], dv[L / 2];
+#pragma unroll
+        for (int q = 0; q < L / 2; ++q) {
+          sv[q] = cur[off + p - q];
+          dv[q] = dl[off + p - q];
+        }
+        inv_pair<T, FILT>(sv, dv, res[k].x, res[k].y);
       }
-      T e0, o0;
-      inv_pair<T, FILT>(sv, dv, e0, o0);
-      *reinterpret_cast<V*>(out + 2 * p) = V{e0, o0};
+    }
+    __syncthreads();  // every s_l read before s_{l-1} overwrites it
+#pragma unroll
+    for (int k = 0; k < kPairs; ++k) {
+      const int p = tid + k * kNT;
+      if (p < npairs) *reinterpret_cast<V*>(cur + 2 * p) = res[k];
     }
     __syncthreads();
   }
-  // bufX holds s_{j0} for local [0, kTile)
+  // the pool holds s_{j0} for local [0, kTile)
   T* __restrict__ dst = static_cast<T*>(a.dst);
   for (int q = 2 * tid; q < kTile; q += 2 * kNT)
-    *reinterpret_cast<V*>(dst + base + q) = *reinterpret_cast<const V*>(bufX + q);
+    *reinterpret_cast<V*>(dst + base + q) = *reinterpret_cast<const V*>(bufS + q);
 }
 
 // ----------------------------------------------------------- whole kernels
