@@ -163,8 +163,14 @@ int smooth(void* x, int64_t n, int J, int ttype, int filt, int dtype, int thresh
   double tc = 0.0;
   if ((rc = d2h(&tc, &sb->vals[0], st))) return rc;
   if ((rc = split_halves(x, xe, xo, n, dtype, st))) return rc;
-  if ((rc = fwd(ttype, xe, We, lenh, Jh, filt, dtype, st))) return rc;
-  if ((rc = fwd(ttype, xo, Wo, lenh, Jh, filt, dtype, st))) return rc;
+  // the two half-length MODWTs run as one two-signal launch (xe, xo and
+  // We, Wo and ye, yo are adjacent): twice the workgroups, half the launches
+  if (ttype == kDWT) {
+    if ((rc = fwd(ttype, xe, We, lenh, Jh, filt, dtype, st))) return rc;
+    if ((rc = fwd(ttype, xo, Wo, lenh, Jh, filt, dtype, st))) return rc;
+  } else if ((rc = modwt_forward(xe, We, lenh, 2, Jh, filt, dtype, st))) {
+    return rc;
+  }
 
   auto eval = [&](double t, double* m) -> int {
     wh_thresh th{};
@@ -174,8 +180,12 @@ int smooth(void* x, int64_t n, int J, int ttype, int filt, int dtype, int thresh
     th.maxlevel = maxlevel - 1;
     th.value = t;
     int r;
-    if ((r = inv(ttype, We, ye, lenh, Jh, filt, dtype, &th, st))) return r;
-    if ((r = inv(ttype, Wo, yo, lenh, Jh, filt, dtype, &th, st))) return r;
+    if (ttype == kDWT) {
+      if ((r = inv(ttype, We, ye, lenh, Jh, filt, dtype, &th, st))) return r;
+      if ((r = inv(ttype, Wo, yo, lenh, Jh, filt, dtype, &th, st))) return r;
+    } else if ((r = modwt_inverse(We, ye, lenh, 2, Jh, filt, dtype, &th, st))) {
+      return r;
+    }
     if ((r = interp_mse(x, ye, yo, n, dtype, sb->partial, &sb->vals[1], st))) return r;
     return d2h(m, &sb->vals[1], st);
   };
