@@ -51,10 +51,27 @@ using KnobKey = std::tuple<int, int64_t, int, int, int>;  // inverse, n, J, filt
 std::mutex g_knob_mu;
 std::map<KnobKey, PlanKnobs> g_knobs;
 
+// Default plan knobs.  An inverse of up to 2^18 samples has too few
+// segments for the whole GPU with one 512-thread workgroup per CU; a
+// 256-thread first group (two per CU, two segments per CU) of at most six
+// levels measured faster than every 512-thread plan there (LA8: 0.025 vs
+// 0.028 ms at n = 2^16, 0.032 vs 0.035 at 2^17, 0.037 vs 0.041 at 2^18);
+// from 2^20 on its deeper groups plan worse and the 512-thread default is
+// faster.
+PlanKnobs default_knobs(bool inverse, int64_t n) {
+  PlanKnobs k;
+  if (inverse && n <= (int64_t(1) << 18)) {
+    k.first_nt = kNT1 / 2;
+    k.first_per_cu = 2;
+    k.g1max = 6;
+  }
+  return k;
+}
+
 PlanKnobs knobs_for(bool inverse, int64_t n, int J, int filt, size_t esz) {
   std::lock_guard<std::mutex> lk(g_knob_mu);
   auto it = g_knobs.find(KnobKey{inverse ? 1 : 0, n, J, filt, int(esz)});
-  return it == g_knobs.end() ? PlanKnobs{} : it->second;
+  return it == g_knobs.end() ? default_knobs(inverse, n) : it->second;
 }
 
 // LDS elements of one residue's buffers (the first inverse group keeps the
@@ -341,7 +358,7 @@ int tune_t(int64_t n, int J, int filt, hipStream_t st) {
     constexpr float kMargin = 0.98f;
     const char* env = std::getenv("WH_TUNE_LOG");  // candidate timings to stderr
     const bool log = env && env[0] == '1';
-    std::vector<PlanKnobs> order{PlanKnobs{}};
+    std::vector<PlanKnobs> order{default_knobs(inverse, n)};
     for (auto& c : cands) order.push_back(c.second);
     float best = 1e30f;
     PlanKnobs bk;
