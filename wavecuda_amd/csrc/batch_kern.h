@@ -64,11 +64,11 @@ __device__ __forceinline__ int batch_next(int* counter, int* sh) {
 }
 
 // phase boundary inside a workgroup when the phases exchange data through
-// global memory (scratch)
-__device__ __forceinline__ void batch_phase() {
-  __threadfence();
-  __syncthreads();
-}
+// global memory (the workgroup's own scratch slice).  Only this
+// workgroup's waves, all on one CU, read what they wrote, so the barrier's
+// workgroup-scope release/acquire is enough; a device-scope fence would
+// write back and invalidate the L2 of the workgroup's XCD at every item.
+__device__ __forceinline__ void batch_phase() { __syncthreads(); }
 
 // MODWT item: fused forward of all J = G levels into scratch (D_j and S_J),
 // MAD of D_1, fused thresholding inverse into y.  Chunk width M = min(2 NT,
