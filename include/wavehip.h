@@ -186,7 +186,8 @@ int wh_threshold_many(void* coeffs, int64_t n, int64_t count, int nlevels, int t
                       int dtype, const wh_thresh* thr, void* stream);
 /* Gradient of the thresholding y = threshold(c): given the coefficients c
    before thresholding and the cotangent gy of y, writes gc (the cotangent
-   of c; gc may equal gy) and, when glam is not NULL, glam[l] (DEVICE, one
+   of c; gc may equal gy, or be NULL when only glam is wanted) and, when
+   glam is not NULL, glam[l] (DEVICE, one
    double per level, J entries) = sum over all signals of dL/dt_l: for the
    soft rule -sign(c) gy over |c| > t_l, for the hard rule 0 (almost
    everywhere); levels outside [minlevel, maxlevel] get 0.  At |c| = t the

@@ -13,6 +13,8 @@ differentiable pipeline.
 * Fitting per-level thresholds by Adam lowers the error below the
   universal threshold's.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 import torch
@@ -200,3 +202,46 @@ def test_fit_level_thresholds_beats_universal():
     print("loss", losses[0], "->", losses[-1], "lam", lam.cpu().numpy(), flush=True)
     assert np.all(np.isfinite(losses))
     assert losses[-1] < losses[0]
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("hardness", [W.SOFT, W.HARD])
+def test_threshold_inverse_matches_composite(dtype, hardness):
+    """The fused threshold + inverse (thresholds applied as the inverse reads)
+    gives the composite threshold-then-inverse's output and gradients bit for
+    bit, in the coefficients and in the thresholds."""
+    B, n, J, f = 3, 1 << 12, 6, W.LA8
+    g = torch.Generator(device="cuda").manual_seed(7)
+    x = torch.randn(B, n, dtype=dtype, device="cuda", generator=g)
+    w = A.modwt(x, J, f).detach()
+    lam = torch.tensor([0.3, 0.25, 0.2, 0.2, 0.15, 0.1], dtype=torch.float64, device="cuda")
+    gy = torch.randn(B, n, dtype=dtype, device="cuda", generator=g)
+    outs = []
+    for fused in (False, True):
+        c = w.clone().requires_grad_(True)
+        lv = lam.clone().requires_grad_(True)
+        if fused:
+            y = A.ThresholdInverseFn.apply(c, lv, n, J, f, hardness, 0, J - 1)
+        else:
+            y = A.ImodwtFn.apply(A.ThresholdFn.apply(c, lv, n, J, W.MODWT, f, hardness, 0, J - 1),
+                                 n, J, f)
+        y.backward(gy)
+        outs.append((y.detach(), c.grad, lv.grad))
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
+
+
+def test_threshold_grad_without_gc():
+    """wh_thresh_grad with gc NULL gives the same per-level gradient."""
+    n, J = 1 << 12, 5
+    c = _cuda(white_noise(2 * n * J, seed=1))
+    gy = _cuda(white_noise(2 * n * J, seed=2))
+    lam = torch.full((J,), 0.5, dtype=torch.float64, device="cuda")
+    thr = W.make_thresh(True, W.SOFT, 0, J - 1, 0.0, lam.data_ptr())
+    gl = [torch.zeros(J, dtype=torch.float64, device="cuda") for _ in range(2)]
+    gc = torch.empty_like(c)
+    for k, gcp in enumerate((gc.data_ptr(), None)):
+        W.check(W.lib().wh_thresh_grad(c.data_ptr(), gy.data_ptr(), gcp, gl[k].data_ptr(), n, 1,
+                                       J, W.MODWT, W.LA8, W.WH_F64, C.byref(thr), None),
+                "wh_thresh_grad")
+    assert torch.equal(gl[0], gl[1])

@@ -135,14 +135,67 @@ class ThresholdFn(torch.autograd.Function):
         c2, lam64 = ctx.saved_tensors
         n, J, ttype, filt, hardness, minlevel, maxlevel, squeeze, lam_dtype = ctx.meta
         g2 = g.contiguous().view_as(c2)
-        gc = torch.empty_like(c2)
-        glam = torch.zeros(J, dtype=torch.float64, device=c2.device)
-        thr = W.make_thresh(True, hardness, minlevel, maxlevel, 0.0, lam64.data_ptr())
-        W.check(W.lib().wh_thresh_grad(_vp(c2), _vp(g2), _vp(gc), _vp(glam), n, c2.shape[0], J,
-                                       ttype, filt, _dtype(c2), C.byref(thr), _stream()),
-                "wh_thresh_grad")
-        return ((gc[0] if squeeze else gc), glam.to(lam_dtype), None, None, None, None, None,
+        gc, glam = _thresh_grads(ctx, c2, g2, lam64, n, J, ttype, filt, hardness, minlevel,
+                                 maxlevel)
+        if gc is not None and squeeze:
+            gc = gc[0]
+        return (gc, None if glam is None else glam.to(lam_dtype), None, None, None, None, None,
                 None, None)
+
+
+def _thresh_grads(ctx, c2, g2, lam64, n, J, ttype, filt, hardness, minlevel, maxlevel):
+    """wh_thresh_grad for the cotangents autograd asks for (ctx inputs 0 and
+    1: the coefficients and the thresholds); None for the others."""
+    want_c, want_lam = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
+    if not (want_c or want_lam):
+        return None, None
+    gc = torch.empty_like(c2) if want_c else None
+    glam = torch.zeros(J, dtype=torch.float64, device=c2.device) if want_lam else None
+    thr = W.make_thresh(True, hardness, minlevel, maxlevel, 0.0, lam64.data_ptr())
+    W.check(W.lib().wh_thresh_grad(_vp(c2), _vp(g2), None if gc is None else _vp(gc),
+                                   None if glam is None else _vp(glam), n, c2.shape[0], J, ttype,
+                                   filt, _dtype(c2), C.byref(thr), _stream()), "wh_thresh_grad")
+    return gc, glam
+
+
+class ThresholdInverseFn(torch.autograd.Function):
+    """y = inverse MODWT of the coefficients with per-level thresholds applied
+    as the inverse reads them (one pass; the same operations as thresholding
+    first, so the same y bit for bit).  Backward: the inverse's adjoint, then
+    the threshold's gradient."""
+
+    @staticmethod
+    def forward(ctx, c, lam, n, J, filt, hardness, minlevel, maxlevel):
+        c2, squeeze = _rows(c.contiguous())
+        B = c2.shape[0]
+        if c2.shape[1] != 2 * n * J:
+            raise W.WaveHipError("coefficient rows must hold 2 n J values")
+        lam64 = lam.detach().to(device=c.device, dtype=torch.float64).contiguous()
+        if lam64.numel() != J:
+            raise W.WaveHipError("one threshold per level expected")
+        thr = W.make_thresh(True, hardness, minlevel, maxlevel, 0.0, lam64.data_ptr())
+        x = torch.empty(B, n, dtype=c.dtype, device=c.device)
+        W.check(W.lib().wh_modwt_inv_many(_vp(c2), _vp(x), n, B, J, filt, _dtype(c), C.byref(thr),
+                                          _stream()), "wh_modwt_inv_many")
+        ctx.save_for_backward(c2, lam64)
+        ctx.meta = (n, J, filt, hardness, minlevel, maxlevel, squeeze, lam.dtype)
+        return x[0] if squeeze else x
+
+    @staticmethod
+    def backward(ctx, g):
+        c2, lam64 = ctx.saved_tensors
+        n, J, filt, hardness, minlevel, maxlevel, squeeze, lam_dtype = ctx.meta
+        B = c2.shape[0]
+        g2 = g.contiguous().view(B, n)
+        gw = torch.empty(B, 2 * n * J, dtype=g.dtype, device=g.device)
+        W.check(W.lib().wh_modwt_inv_adj(_vp(g2), _vp(gw), n, B, J, filt, _dtype(g2), _stream()),
+                "wh_modwt_inv_adj")
+        gc, glam = _thresh_grads(ctx, c2, gw, lam64, n, J, W.MODWT, filt, hardness, minlevel,
+                                 maxlevel)
+        if gc is not None and squeeze:
+            gc = gc[0]
+        return (gc, None if glam is None else glam.to(lam_dtype), None, None, None, None, None,
+                None)
 
 
 def modwt(x, nlevels, filter="LA8"):
@@ -187,8 +240,7 @@ def smooth_levels(x, lam, nlevels, filter="LA8", ttype=W.MODWT, hardness=W.SOFT)
     J = resolve_levels(n, nlevels, f)
     if ttype == W.MODWT:
         w = ModwtFn.apply(x, J, f)
-        return ImodwtFn.apply(ThresholdFn.apply(w, lam, n, J, W.MODWT, f, hardness, 0, J - 1),
-                              n, J, f)
+        return ThresholdInverseFn.apply(w, lam, n, J, f, hardness, 0, J - 1)
     y = DwtFn.apply(x, J, f)
     return IdwtFn.apply(ThresholdFn.apply(y, lam, n, J, W.DWT, f, hardness, 0, J - 1), J, f)
 
@@ -213,15 +265,24 @@ def fit_level_thresholds(x, clean, nlevels, filter="LA8", lam0=None, steps=300, 
         lam0 = torch.full((J,), t, dtype=torch.float64)
     lam = lam0.detach().to(device=x.device, dtype=torch.float64).clone().requires_grad_(True)
     opt = torch.optim.Adam([lam], lr=lr)
-    losses = []
     xd = x.detach()
+    if ttype == W.MODWT:
+        # the signals do not change: transform them once, and let each step
+        # run the thresholding inverse and its adjoint only
+        w = ModwtFn.apply(xd, J, f)
+
+        def smooth(lv):
+            return ThresholdInverseFn.apply(w, lv, n, J, f, W.SOFT, 0, J - 1)
+    else:
+        def smooth(lv):
+            return smooth_levels(xd, lv, J, f, ttype)
+    losses = []  # kept on the device: no host round trip per step
     for _ in range(steps):
         opt.zero_grad(set_to_none=True)
-        y = smooth_levels(xd, lam, J, f, ttype)
-        loss = torch.mean((y - clean) ** 2)
+        loss = torch.mean((smooth(lam) - clean) ** 2)
         loss.backward()
         opt.step()
-        losses.append(loss.item())
+        losses.append(loss.detach())
     with torch.no_grad():
-        losses.append(torch.mean((smooth_levels(xd, lam, J, f, ttype) - clean) ** 2).item())
-    return lam.detach(), losses
+        losses.append(torch.mean((smooth(lam) - clean) ** 2))
+    return lam.detach(), torch.stack(losses).double().cpu().tolist()

@@ -106,14 +106,14 @@ __global__ __launch_bounds__(kGradNT) void thresh_grad_k(const T* __restrict__ c
                                                          const T* __restrict__ gy0,
                                                          T* __restrict__ gc0,
                                                          double* __restrict__ partial, int64_t n,
-                                                         int J, int ttype, Thr1 th) {
+                                                         int J, int ttype, Thr1 th, int write_gc) {
   const int l = th.minlevel + int(blockIdx.y);
   const int nl = int(gridDim.y);
   const int64_t sig = blockIdx.z;
   const int64_t per = (ttype == kDWT) ? n : 2 * n * J;
   const T* __restrict__ c = c0 + sig * per;
   const T* __restrict__ gy = gy0 + sig * per;
-  T* __restrict__ gc = gc0 + sig * per;
+  T* __restrict__ gc = write_gc ? gc0 + sig * per : nullptr;
   int64_t off, stride, cnt;
   if (ttype == kDWT) {
     off = int64_t(1) << l;
@@ -135,9 +135,9 @@ __global__ __launch_bounds__(kGradNT) void thresh_grad_k(const T* __restrict__ c
     const T a = fabs(cv);
     if (th.hardness == kSOFT) {
       const bool pass = a > t;
-      gc[i] = pass ? g : T(0);
+      if (write_gc) gc[i] = pass ? g : T(0);
       if (pass) acc += double(cv < T(0) ? g : -g);
-    } else {
+    } else if (write_gc) {
       gc[i] = (a < t) ? T(0) : g;
     }
   }
@@ -192,7 +192,9 @@ int thresh_grad(const void* c, const void* gy, void* gc, double* glam, int64_t n
   if (count < 1 || count > 65535) return set_error(WH_ERR_ARG, "threshold gradient: signal count out of range");
   const size_t esz = dtype == WH_F32 ? 4 : 8;
   const int64_t per = (ttype == kDWT) ? n : 2 * n * J;
-  if (gc != gy)
+  // gc takes gy where the threshold does not act (scaling coefficients,
+  // levels outside the range); without gc only glam is computed
+  if (gc && gc != gy)
     WH_HIP_CHECK(hipMemcpyAsync(gc, gy, size_t(per) * size_t(count) * esz,
                                 hipMemcpyDeviceToDevice, st));
   const int nl = thr->maxlevel - thr->minlevel + 1;
@@ -206,11 +208,11 @@ int thresh_grad(const void* c, const void* gy, void* gc, double* glam, int64_t n
   if (dtype == WH_F32)
     hipLaunchKernelGGL(thresh_grad_k<float>, grid, dim3(kGradNT), 0, st,
                        static_cast<const float*>(c), static_cast<const float*>(gy),
-                       static_cast<float*>(gc), partial, n, J, ttype, th);
+                       static_cast<float*>(gc), partial, n, J, ttype, th, gc ? 1 : 0);
   else
     hipLaunchKernelGGL(thresh_grad_k<double>, grid, dim3(kGradNT), 0, st,
                        static_cast<const double*>(c), static_cast<const double*>(gy),
-                       static_cast<double*>(gc), partial, n, J, ttype, th);
+                       static_cast<double*>(gc), partial, n, J, ttype, th, gc ? 1 : 0);
   WH_HIP_CHECK(hipGetLastError());
   if (glam) {
     hipLaunchKernelGGL(thresh_grad_reduce_k, dim3(unsigned(J)), dim3(kGradNT), 0, st, partial,

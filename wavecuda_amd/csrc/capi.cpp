@@ -272,7 +272,9 @@ int wh_thresh_grad(const void* c, const void* gy, void* gc, double* glam, int64_
                    const wh_thresh* thr, void* stream) {
   int J, rc = check_common(c, gy, n, nlevels, filter, dtype, &J);
   if (rc || (rc = check_count(count))) return rc;
-  if (!gc) return set_error(WH_ERR_ARG, "null pointer");
+  if (!gc && !glam) return set_error(WH_ERR_ARG, "threshold gradient: gc and glam both NULL");
+  if (gc && (reinterpret_cast<uintptr_t>(gc) & (dtype == WH_F64 ? 15 : 7)))
+    return set_error(WH_ERR_ARG, "device buffers must be aligned to two elements (16 bytes FP64, 8 bytes FP32)");
   if (ttype != kDWT && ttype != kMODWT) return set_error(WH_ERR_ARG, "ttype must be 0 or 1");
   if (!thr || !thr->enabled) return set_error(WH_ERR_ARG, "threshold gradient needs a threshold");
   if ((rc = check_thresh(thr, J))) return rc;
