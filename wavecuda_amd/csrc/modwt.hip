@@ -51,16 +51,17 @@ using KnobKey = std::tuple<int, int64_t, int, int, int>;  // inverse, n, J, filt
 std::mutex g_knob_mu;
 std::map<KnobKey, PlanKnobs> g_knobs;
 
-// Default plan knobs.  An inverse of up to 2^18 samples has too few
-// segments for the whole GPU with one 512-thread workgroup per CU; a
+// Default plan knobs.  An inverse launch of up to 2^18 samples (length
+// times signals) has too few segments for the whole GPU with one
+// 512-thread workgroup per CU; a
 // 256-thread first group (two per CU, two segments per CU) of at most six
 // levels measured faster than every 512-thread plan there (LA8: 0.025 vs
 // 0.028 ms at n = 2^16, 0.032 vs 0.035 at 2^17, 0.037 vs 0.041 at 2^18);
 // from 2^20 on its deeper groups plan worse and the 512-thread default is
 // faster.
-PlanKnobs default_knobs(bool inverse, int64_t n) {
+PlanKnobs default_knobs(bool inverse, int64_t n, int64_t count) {
   PlanKnobs k;
-  if (inverse && n <= (int64_t(1) << 18)) {
+  if (inverse && n * count <= (int64_t(1) << 18)) {
     k.first_nt = kNT1 / 2;
     k.first_per_cu = 2;
     k.g1max = 6;
@@ -68,10 +69,10 @@ PlanKnobs default_knobs(bool inverse, int64_t n) {
   return k;
 }
 
-PlanKnobs knobs_for(bool inverse, int64_t n, int J, int filt, size_t esz) {
+PlanKnobs knobs_for(bool inverse, int64_t n, int J, int filt, size_t esz, int64_t count = 1) {
   std::lock_guard<std::mutex> lk(g_knob_mu);
   auto it = g_knobs.find(KnobKey{inverse ? 1 : 0, n, J, filt, int(esz)});
-  return it == g_knobs.end() ? default_knobs(inverse, n) : it->second;
+  return it == g_knobs.end() ? default_knobs(inverse, n, count) : it->second;
 }
 
 // LDS elements of one residue's buffers (the first inverse group keeps the
@@ -89,7 +90,7 @@ int64_t round_up64(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 // budget allows while its warm-up stays at most half a segment (one
 // segment per CU for the first group, kDeepSegPerCu for deeper ones).
 std::vector<Group> plan_groups(bool inverse, int64_t n, int J, int L, size_t esz,
-                               const PlanKnobs& k) {
+                               const PlanKnobs& k, int64_t count = 1) {
   const DeviceInfo& dev = device_info();
   const size_t budget = std::min<size_t>(dev.max_lds_per_block, 160 * 1024) - 1024;
   const int64_t cus = std::max(1, dev.num_cus);
@@ -106,8 +107,10 @@ std::vector<Group> plan_groups(bool inverse, int64_t n, int J, int L, size_t esz
     const int64_t Nm = n >> j0;
     G.M = int(std::min<int64_t>(2 * G.NT / G.R, Nm));
     const int64_t nrb = B / G.R;
-    const int64_t target =
-        std::max<int64_t>(1, (j0 == 0 ? k.first_per_cu : k.deep_per_cu) * cus / nrb);
+    // segments per signal: the CUs' worth of workgroups is shared by the
+    // `count` signals of a many-signal launch
+    const int64_t target = std::max<int64_t>(
+        1, (j0 == 0 ? k.first_per_cu : k.deep_per_cu) * cus / (nrb * std::max<int64_t>(1, count)));
     int64_t nseg_t = 1;
     while (nseg_t * 2 <= target && Nm / (nseg_t * 2) >= G.M) nseg_t *= 2;
     const int64_t Pt = Nm / nseg_t;
@@ -195,7 +198,8 @@ template <typename T>
 int modwt_fwd_t(const void* x, void* xmod, int64_t n, int64_t count, int J, int filt, int sp,
                 hipStream_t st) {
   const int L = filter_length(filt);
-  auto groups = plan_groups(false, n, J, L, sizeof(T), knobs_for(false, n, J, filt, sizeof(T)));
+  auto groups = plan_groups(false, n, J, L, sizeof(T),
+                            knobs_for(false, n, J, filt, sizeof(T), count), count);
   const int64_t per = 2 * n * J;
   for (const Group& G : groups) {
     ModwtArgs a = args_of(G, n);
@@ -225,7 +229,8 @@ template <typename T>
 int modwt_inv_t(const void* xmod, void* x, int64_t n, int64_t count, int J, int filt,
                 const wh_thresh* thr, hipStream_t st) {
   const int L = filter_length(filt);
-  auto groups = plan_groups(true, n, J, L, sizeof(T), knobs_for(true, n, J, filt, sizeof(T)));
+  auto groups = plan_groups(true, n, J, L, sizeof(T), knobs_for(true, n, J, filt, sizeof(T), count),
+                            count);
   const int64_t per = 2 * n * J;
   // the S_{j0} handed between groups lives in library scratch (2 n values
   // per signal: two halves used alternately)
@@ -358,7 +363,7 @@ int tune_t(int64_t n, int J, int filt, hipStream_t st) {
     constexpr float kMargin = 0.98f;
     const char* env = std::getenv("WH_TUNE_LOG");  // candidate timings to stderr
     const bool log = env && env[0] == '1';
-    std::vector<PlanKnobs> order{default_knobs(inverse, n)};
+    std::vector<PlanKnobs> order{default_knobs(inverse, n, 1)};
     for (auto& c : cands) order.push_back(c.second);
     float best = 1e30f;
     PlanKnobs bk;
