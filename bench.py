@@ -389,6 +389,7 @@ def diff_section(steps=300):
     xs = clean[None, :] + (sd / snr)[:, None] * rng.standard_normal((B, n))
     x = torch.tensor(xs, dtype=torch.float32, device="cuda")
     cl = torch.tensor(np.broadcast_to(clean, (B, n)).copy(), dtype=torch.float32, device="cuda")
+    A.fit_level_thresholds(x, cl, J, W.LA8, steps=3, lr=1e-2)  # untimed warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     lam, losses = A.fit_level_thresholds(x, cl, J, W.LA8, steps=steps, lr=1e-2)
