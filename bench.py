@@ -53,6 +53,22 @@ def modwt_bytes_inv(n, J, esz=8):
     return esz * n * (J + 2)
 
 
+def plan_kernels(plan: str, inverse: bool) -> str:
+    """The fused MODWT kernels a plan string launches, in launch order
+    (LA8, FP64): modwt_fwd_k<T, filter, R, threads, levels, map, store> /
+    modwt_inv_k<T, filter, R, threads, levels, map>."""
+    import re
+
+    groups = re.findall(r"\{j0=(\d+) g=(\d+) R=(\d+) NT=(\d+)", plan)
+    names = []
+    for _, g, r, nt in (reversed(groups) if inverse else groups):
+        if inverse:
+            names.append(f"modwt_inv_k<double, 4, {r}, {nt}, {g}, {2 if r == '1' else 0}>")
+        else:
+            names.append(f"modwt_fwd_k<double, 4, {r}, {nt}, {g}, 0, 0>")
+    return " then ".join(names)
+
+
 def traffic_from_profiles():
     """HBM bytes per launch of the forward kernel measured by rocprofv3
     FETCH_SIZE / WRITE_SIZE (scaled by a copy-kernel calibration), if the
@@ -518,6 +534,7 @@ def main():
     del junk
     cold_ms = sorted(cold)[len(cold) // 2]
 
+    fwd_plan, inv_plan = D.modwt_plan(False, n, J, filt), D.modwt_plan(True, n, J, filt)
     value = n / (ms_per_step * 1e-3) / 1e6
     bf, bi = modwt_bytes_fwd(n, J), modwt_bytes_inv(n, J)
     fwd_gbs = bf / (fwd_ms * 1e-3) / 1e9
@@ -547,7 +564,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "modwt_fwd_k<double, 4, 1, 512, 10, 0, 0> (LA8, R=1, 512 threads, G=10)",
+            "kernel": plan_kernels(fwd_plan, False),
             "achieved": round(fwd_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -557,7 +574,7 @@ def main():
             "avg_launch_ms": round(fwd_ms, 4),
         },
         "inverse_kernel": {
-            "kernel": "modwt_inv_k<double, 4, 32, 256, 2, 0> then modwt_inv_k<double, 4, 1, 512, 8, 2>",
+            "kernel": plan_kernels(inv_plan, True),
             "achieved_GBs": round(inv_gbs, 1),
             "frac": round(inv_gbs / HBM_PEAK_GBS, 4),
             "algorithmic_bytes_per_launch": bi,
@@ -566,8 +583,7 @@ def main():
         "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
         "cold_ms_per_step": round(cold_ms, 4),
         "cold_msamples_per_s": round(n / (cold_ms * 1e-3) / 1e6, 2),
-        "plan": {"fwd": D.modwt_plan(False, n, J, filt), "inv": D.modwt_plan(True, n, J, filt),
-                 "tuned": not args.no_tune},
+        "plan": {"fwd": fwd_plan, "inv": inv_plan, "tuned": not args.no_tune},
         "reconstruction_relerr": err,
     }
     if not args.no_batch:
