@@ -61,6 +61,26 @@ def test_median_mad_exact(n):
     assert D.mad(g).item() == O.mad(x)
 
 
+@pytest.mark.parametrize("case", ["exact_ties", "crowded_bucket", "past_2p22"])
+def test_median_mad_candidate_overflow(case):
+    """More keys under a middle rank's leading digits than the finishing
+    workgroup holds: a bucket of one repeated value (resolved from the
+    bucket's min = max), a bucket of 1000 distinct values within 2^-30 of
+    each other (resolved over the data), and 2^22 + 5 values (three grid
+    digits before the compaction)."""
+    rng = np.random.Generator(np.random.PCG64(7))
+    if case == "exact_ties":
+        x = rng.integers(-1, 3, 1 << 20).astype(np.float64)
+    elif case == "crowded_bucket":
+        x = 1.0 + rng.integers(0, 1000, 1 << 20) * 2.0 ** -40
+    else:
+        x = np.round(rng.standard_normal((1 << 22) + 5) * 64) / 64
+    print(f"median overflow {case} n={x.size}", flush=True)
+    g = _dev(x)
+    assert D.median(g).item() == O.median(x)
+    assert D.mad(g).item() == O.mad(x)
+
+
 def test_median_special_values():
     x = np.array([-0.0, 0.0, -1e-300, 1e300, -np.inf, np.inf, 5e-324, -5e-324, 2.5])
     g = _dev(x)

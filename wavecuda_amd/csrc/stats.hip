@@ -4,9 +4,11 @@
 //    (reference src/thresh.cpp:15-83, :142-157);
 //  * exact median and MAD (reference src/utils.cpp:166-203) by MSB-first
 //    radix selection over order-preserving integer keys of the values:
-//    each pass builds an LDS-privatised histogram of one 11-bit digit of
-//    the keys that still match the prefix found so far, and a one-block
-//    scan picks the bucket holding the wanted rank.  Both middle order
+//    each grid pass builds an LDS-privatised histogram of one 11-bit digit
+//    of the keys that still match the prefix found so far, and a one-block
+//    scan picks the bucket holding the wanted rank; for FP64 keys the few
+//    keys left after the leading digits are compacted and one workgroup
+//    resolves the remaining digits in LDS.  Both middle order
 //    statistics are tracked at once; results are bit-exact (a median is an
 //    order statistic; the final 0.5(a+b) and 1.4826 m are the reference's
 //    operations);
@@ -17,6 +19,7 @@
 //    fixed-order two-stage reduction, so repeated runs are bit-identical.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "wh_common.h"
 #include "wh_internal.h"
@@ -139,6 +142,9 @@ __global__ void sel_init_kernel(SelState* st, int64_t count) {
     st->prefix[0] = st->prefix[1] = 0;
     st->k[0] = (count - 1) / 2;
     st->k[1] = count / 2;
+    st->ncand = 0;
+    st->cmin[0] = st->cmin[1] = ~0ull;
+    st->cmax[0] = st->cmax[1] = 0;
   }
 }
 
@@ -219,38 +225,252 @@ __global__ __launch_bounds__(256) void sel_scan_kernel(SelState* st, int hi, int
   for (int i = t; i < 2 * kBins; i += 256) st->hist[i] = 0;
 }
 
+// After the grid passes: every key whose bits above `hi` equal either
+// prefix is appended to the candidate buffer, one atomic per wave.  Past
+// the buffer's capacity a wave stops appending (the count then exceeds
+// the capacity and the finishing workgroup reads the data itself); the
+// smallest and largest key under each prefix are kept either way, so a
+// bucket of exact ties needs no further pass.
 template <typename T>
-__global__ void sel_finish_kernel(SelState* st, int deviation, double factor, double* out) {
-  if (threadIdx.x != 0) return;
-  const double a = double(value_from_key<T>(st->prefix[0]));
-  const double b = double(value_from_key<T>(st->prefix[1]));
-  const double med = 0.5 * (a + b);
-  if (!deviation) {
-    st->med = med;
-    if (out) *out = med;
-  } else {
-    st->mad = 1.4826 * med;
-    if (out) *out = (factor > 0.0) ? st->mad * factor : st->mad;
+__global__ __launch_bounds__(256) void sel_compact_kernel(SelSrc s, SelState* st, int hi,
+                                                          int deviation) {
+  __shared__ uint64_t red[4][4];
+  const uint64_t p0 = st->prefix[0], p1 = st->prefix[1];
+  const T med = deviation ? T(st->med) : T(0);
+  const int64_t E = (s.mode == 2) ? s.n : s.count;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t mn0 = ~0ull, mx0 = 0, mn1 = ~0ull, mx1 = 0;
+  bool full = false;
+  for (int64_t base = int64_t(blockIdx.x) * blockDim.x; base < E; base += stride) {
+    const int64_t e = base + threadIdx.x;
+    bool c = false;
+    uint64_t k = 0;
+    T v;
+    if (e < E && sel_value<T>(s, e, v)) {
+      if (deviation) v = fabs(v - med);
+      k = okey(v);
+      const uint64_t top = k >> hi;
+      if (top == p0) {
+        mn0 = k < mn0 ? k : mn0;
+        mx0 = k > mx0 ? k : mx0;
+        c = true;
+      }
+      if (top == p1) {
+        mn1 = k < mn1 ? k : mn1;
+        mx1 = k > mx1 ? k : mx1;
+        c = true;
+      }
+    }
+    if (full) continue;
+    const uint64_t mask = __ballot(c);
+    if (!mask) continue;
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    const unsigned cnt = unsigned(__popcll(mask));
+    unsigned long long old = 0;
+    if (lane == leader) old = atomicAdd(&st->ncand, (unsigned long long)cnt);
+    old = __shfl(old, leader, 64);
+    if (c) {
+      const unsigned long long idx = old + __popcll(mask & ((1ull << lane) - 1ull));
+      if (idx < (unsigned long long)kSelCandCap) st->cand[idx] = k;
+    }
+    full = old + cnt > (unsigned long long)kSelCandCap;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t a0 = __shfl_xor(mn0, o, 64), b0 = __shfl_xor(mx0, o, 64);
+    const uint64_t a1 = __shfl_xor(mn1, o, 64), b1 = __shfl_xor(mx1, o, 64);
+    mn0 = a0 < mn0 ? a0 : mn0;
+    mx0 = b0 > mx0 ? b0 : mx0;
+    mn1 = a1 < mn1 ? a1 : mn1;
+    mx1 = b1 > mx1 ? b1 : mx1;
+  }
+  if (lane == 0) {
+    red[wave][0] = mn0;
+    red[wave][1] = mx0;
+    red[wave][2] = mn1;
+    red[wave][3] = mx1;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int j = threadIdx.x;
+    uint64_t r = red[0][j];
+    for (int w = 1; w < 4; ++w) r = (j & 1) ? (red[w][j] > r ? red[w][j] : r)
+                                            : (red[w][j] < r ? red[w][j] : r);
+    if (j & 1) {
+      if (r != 0) atomicMax((unsigned long long*)&st->cmax[j >> 1], (unsigned long long)r);
+    } else {
+      if (r != ~0ull) atomicMin((unsigned long long*)&st->cmin[j >> 1], (unsigned long long)r);
+    }
   }
 }
 
+constexpr int kFinNT = 1024;
+constexpr size_t kFinLds = size_t(kSelCandCap) * 8 + size_t(2) * kBins * 4;
+
+// inclusive prefix sum of v over the workgroup (kFinNT threads): within
+// each wave by shuffles, then over the wave totals
+__device__ __forceinline__ unsigned fin_scan(unsigned v, unsigned* wsum) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  if (lane == 63) wsum[wave] = v;
+  __syncthreads();
+  if (wave == 0) {
+    unsigned w = lane < kFinNT / 64 ? wsum[lane] : 0u;
+#pragma unroll
+    for (int o = 1; o < kFinNT / 64; o <<= 1) {
+      const unsigned t = __shfl_up(w, o, 64);
+      if (lane >= o) w += t;
+    }
+    if (lane < kFinNT / 64) wsum[lane] = w;
+  }
+  __syncthreads();
+  return v + (wave ? wsum[wave - 1] : 0u);
+}
+
+// One workgroup resolves the digits below `hi0` from the candidate keys in
+// LDS (or, when they overflowed the buffer, from the data itself) and
+// forms the median 0.5 (a + b) -- or the MAD 1.4826 median (times `factor`
+// when > 0).
+template <typename T>
+__global__ __launch_bounds__(kFinNT) void sel_finish_kernel(SelSrc s, SelState* st, int hi0,
+                                                            int deviation, double factor,
+                                                            double* out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint64_t* cand = reinterpret_cast<uint64_t*>(smem);
+  unsigned* hist = reinterpret_cast<unsigned*>(cand + kSelCandCap);  // [2][kBins]
+  __shared__ unsigned wsum[kFinNT / 64];
+  __shared__ uint64_t pre[2];
+  __shared__ long long ks[2];
+  __shared__ int dsel[2];
+  __shared__ long long below[2];
+  const int tid = threadIdx.x;
+  const unsigned long long nall = hi0 > 0 ? st->ncand : 0;
+  const bool in_lds = nall <= (unsigned long long)kSelCandCap;
+  const int nc = in_lds ? int(nall) : 0;
+  // overflow with each bucket holding a single key value: those are the keys
+  const bool ties = !in_lds && st->cmin[0] == st->cmax[0] && st->cmin[1] == st->cmax[1];
+  if (tid == 0) {
+    pre[0] = ties ? st->cmin[0] : st->prefix[0];
+    pre[1] = ties ? st->cmin[1] : st->prefix[1];
+    ks[0] = st->k[0];
+    ks[1] = st->k[1];
+  }
+  for (int i = tid; i < nc; i += kFinNT) cand[i] = st->cand[i];
+  const T med = deviation ? T(st->med) : T(0);
+  const int64_t E = (s.mode == 2) ? s.n : s.count;
+  __syncthreads();
+  for (int hi = ties ? 0 : hi0; hi > 0;) {
+    const int lo = hi > kBits ? hi - kBits : 0;
+    for (int i = tid; i < 2 * kBins; i += kFinNT) hist[i] = 0;
+    __syncthreads();
+    const uint64_t p0 = pre[0], p1 = pre[1];
+    const bool same = (p0 == p1);
+    const uint64_t dmask = (uint64_t(1) << (hi - lo)) - 1;
+    auto count_key = [&](uint64_t k) {
+      const uint64_t top = k >> hi;
+      const unsigned d = unsigned((k >> lo) & dmask);
+      if (top == p0) atomicAdd(&hist[d], 1u);
+      if (!same && top == p1) atomicAdd(&hist[kBins + d], 1u);
+    };
+    if (in_lds) {
+      for (int i = tid; i < nc; i += kFinNT) count_key(cand[i]);
+    } else {
+      for (int64_t e = tid; e < E; e += kFinNT) {
+        T v;
+        if (!sel_value<T>(s, e, v)) continue;
+        if (deviation) v = fabs(v - med);
+        count_key(okey(v));
+      }
+    }
+    __syncthreads();
+    constexpr int per = kBins / kFinNT;
+    for (int r = 0; r < 2; ++r) {
+      const unsigned* hr = hist + (same ? 0 : r) * kBins;
+      const long long k = ks[r];
+      unsigned loc = 0;
+#pragma unroll
+      for (int i = 0; i < per; ++i) loc += hr[tid * per + i];
+      const unsigned incl = fin_scan(loc, wsum);
+      const long long before = (long long)(incl - loc);
+      if (before <= k && k < (long long)incl) {
+        long long acc = before;
+        for (int i = 0; i < per; ++i) {
+          const unsigned c = hr[tid * per + i];
+          if (acc + c > k) {
+            dsel[r] = tid * per + i;
+            below[r] = acc;
+            break;
+          }
+          acc += c;
+        }
+      }
+      __syncthreads();
+      if (tid == 0) {
+        ks[r] = k - below[r];
+        pre[r] = (pre[r] << (hi - lo)) | uint64_t(dsel[r]);
+      }
+      __syncthreads();
+    }
+    hi = lo;
+  }
+  if (tid == 0) {
+    const double a = double(value_from_key<T>(pre[0]));
+    const double b = double(value_from_key<T>(pre[1]));
+    const double m = 0.5 * (a + b);
+    if (!deviation) {
+      st->med = m;
+      if (out) *out = m;
+    } else {
+      st->mad = 1.4826 * m;
+      if (out) *out = (factor > 0.0) ? st->mad * factor : st->mad;
+    }
+  }
+}
+
+// Number of 11-bit digits resolved by grid passes before the keys still
+// in play are compacted: FP32 keys (three digits) take every digit on the
+// grid -- compaction would save no pass -- and FP64 keys take two digits up
+// to 2^22 values and three beyond, so that the bucket of a middle rank
+// holds about 2^-10 (2^-21) of the values around it and fits the
+// finishing workgroup's LDS.
+inline int grid_digits(int key_bits, int64_t E) {
+  if (key_bits == 32 || E >= (int64_t(1) << 31)) return (key_bits + kBits - 1) / kBits;
+  return E <= (int64_t(1) << 22) ? 2 : 3;
+}
+
+// Grid histogram passes over the leading digits, then (FP64) a compaction
+// of the keys still in play and one workgroup that resolves the remaining
+// digits in LDS: seven or nine launches per FP64 median instead of
+// thirteen, three or four passes over the values instead of six.
 template <typename T>
 int select_median_t(const SelSrc& s, bool deviation, SelState* st, double factor, double* out,
                     hipStream_t stream) {
   const int key_bits = sizeof(T) * 8;
+  static std::once_flag once;
+  std::call_once(once, [] { allow_dynamic_lds(reinterpret_cast<const void*>(sel_finish_kernel<T>)); });
   hipLaunchKernelGGL(sel_init_kernel, dim3(1), dim3(256), 0, stream, st, s.count);
   const int64_t E = (s.mode == 2) ? s.n : s.count;
   const int cus = std::max(1, device_info().num_cus);
   const unsigned blocks = unsigned(std::max<int64_t>(1, std::min<int64_t>((E + 2047) / 2048, 4 * cus)));
-  for (int hi = key_bits; hi > 0;) {
-    const int lo = std::max(0, hi - kBits);
+  const int gd = grid_digits(key_bits, E);
+  int hi = key_bits;
+  for (int d = 0; d < gd; ++d) {
+    const int lo = hi > kBits ? hi - kBits : 0;
     hipLaunchKernelGGL(sel_hist_kernel<T>, dim3(blocks), dim3(256), 0, stream, s, st, hi, lo,
                        deviation ? 1 : 0);
     hipLaunchKernelGGL(sel_scan_kernel, dim3(1), dim3(256), 0, stream, st, hi, lo);
     hi = lo;
   }
-  hipLaunchKernelGGL(sel_finish_kernel<T>, dim3(1), dim3(64), 0, stream, st,
-                     deviation ? 1 : 0, factor, out);
+  if (hi > 0)
+    hipLaunchKernelGGL(sel_compact_kernel<T>, dim3(blocks), dim3(256), 0, stream, s, st, hi,
+                       deviation ? 1 : 0);
+  hipLaunchKernelGGL(sel_finish_kernel<T>, dim3(1), dim3(kFinNT), hi > 0 ? kFinLds : 0, stream,
+                     s, st, hi, deviation ? 1 : 0, factor, out);
   WH_HIP_CHECK(hipGetLastError());
   return WH_OK;
 }

@@ -31,12 +31,17 @@ struct SelSrc {
 };
 
 // device-resident selection state
+constexpr int kSelCandCap = 16384;  // keys the finishing workgroup keeps in LDS
+
 struct SelState {
   uint64_t prefix[2];
   long long k[2];
   double med;
   double mad;
+  unsigned long long ncand;  // keys appended to cand by the compaction pass
+  uint64_t cmin[2], cmax[2];  // smallest / largest key under each prefix
   unsigned long long hist[2 * 2048];
+  uint64_t cand[kSelCandCap];
 };
 
 constexpr int kMaxPartials = 1024;
