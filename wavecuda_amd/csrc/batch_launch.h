@@ -12,8 +12,7 @@ namespace wh {
 template <typename T, int NT>
 int batch_launch_nt(const BatchLaunch& bl, const BatchArgs& a, hipStream_t st) {
   auto kern = batch_smooth_k<T, NT>;
-  static std::once_flag once;
-  std::call_once(once, [&] { allow_dynamic_lds(reinterpret_cast<const void*>(kern)); });
+  allow_dynamic_lds(reinterpret_cast<const void*>(kern));
   hipLaunchKernelGGL(kern, dim3(bl.grid), dim3(NT), bl.lds, st, a);
   WH_HIP_CHECK(hipGetLastError());
   return WH_OK;

@@ -7,7 +7,9 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <set>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "wh_common.h"
@@ -22,6 +24,7 @@ std::mutex g_mu;
 
 constexpr int kMaxDev = 64;
 DeviceInfo g_info[kMaxDev];
+std::set<std::pair<int, const void*>> g_lds_set;  // (device, kernel) pairs already set
 struct Scratch {
   void* p = nullptr;
   size_t bytes = 0;
@@ -91,6 +94,10 @@ const DeviceInfo& device_info() {
 }
 
 void allow_dynamic_lds(const void* kern) {
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_lds_set.insert({current_device(), kern}).second) return;
+  }
   hipFuncAttributes at{};
   size_t stat = 0;
   if (hipFuncGetAttributes(&at, kern) == hipSuccess) stat = at.sharedSizeBytes;

@@ -379,8 +379,7 @@ int64_t whole_max() {
 
 template <typename KERN>
 void allow_big_lds(KERN kern) {
-  static std::once_flag once;
-  std::call_once(once, [&] { allow_dynamic_lds(reinterpret_cast<const void*>(kern)); });
+  allow_dynamic_lds(reinterpret_cast<const void*>(kern));
 }
 
 template <typename T>

@@ -11,8 +11,7 @@ namespace wh {
 
 template <typename KERN>
 inline void modwt_allow_lds(KERN kern) {
-  static std::once_flag once;
-  std::call_once(once, [&] { allow_dynamic_lds(reinterpret_cast<const void*>(kern)); });
+  allow_dynamic_lds(reinterpret_cast<const void*>(kern));
 }
 
 template <typename T, int FILT, int R, int NT, int G>

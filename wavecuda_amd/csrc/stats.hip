@@ -451,8 +451,7 @@ template <typename T>
 int select_median_t(const SelSrc& s, bool deviation, SelState* st, double factor, double* out,
                     hipStream_t stream) {
   const int key_bits = sizeof(T) * 8;
-  static std::once_flag once;
-  std::call_once(once, [] { allow_dynamic_lds(reinterpret_cast<const void*>(sel_finish_kernel<T>)); });
+  allow_dynamic_lds(reinterpret_cast<const void*>(sel_finish_kernel<T>));
   hipLaunchKernelGGL(sel_init_kernel, dim3(1), dim3(256), 0, stream, st, s.count);
   const int64_t E = (s.mode == 2) ? s.n : s.count;
   const int cus = std::max(1, device_info().num_cus);

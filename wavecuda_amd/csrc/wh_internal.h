@@ -33,9 +33,10 @@ int scratch2_get(size_t bytes, void** out);
 int ws_get(size_t bytes, void** out);
 int stats_get(size_t bytes, void** out);
 
-// Let kernel `kern` take all of the device's LDS beyond its static LDS as
-// dynamic LDS.  A refused request is cleared from HIP's last error so it
-// cannot surface in a later, unrelated call.
+// Let kernel `kern` take all of the current device's LDS beyond its static
+// LDS as dynamic LDS; done once per (device, kernel), so it is cheap to call
+// before every launch.  A refused request is cleared from HIP's last error
+// so it cannot surface in a later, unrelated call.
 void allow_dynamic_lds(const void* kern);
 
 // transform back ends (device pointers)
