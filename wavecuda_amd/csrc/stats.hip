@@ -332,10 +332,16 @@ __device__ __forceinline__ unsigned fin_scan(unsigned v, unsigned* wsum) {
   return v + (wave ? wsum[wave - 1] : 0u);
 }
 
-// One workgroup resolves the digits below `hi0` from the candidate keys in
-// LDS (or, when they overflowed the buffer, from the data itself) and
-// forms the median 0.5 (a + b) -- or the MAD 1.4826 median (times `factor`
-// when > 0).
+// One workgroup resolves the digits below `hi0` and forms the median
+// 0.5 (a + b) -- or the MAD 1.4826 median (times `factor` when > 0).  The
+// keys under each middle rank's prefix come from the candidate buffer,
+// held in LDS: up to kFinCount of them are ranked by direct counting;
+// more take 11-bit digits, both ranks' bucket counts packed in one word
+// (each at most kSelCandCap < 2^16) so that one scan serves both.  When
+// the buffer overflowed, a bucket of one repeated value is that value, and
+// otherwise the workgroup reads the values themselves.
+constexpr int kFinCount = 256;
+
 template <typename T>
 __global__ __launch_bounds__(kFinNT) void sel_finish_kernel(SelSrc s, SelState* st, int hi0,
                                                             int deviation, double factor,
@@ -354,6 +360,7 @@ __global__ __launch_bounds__(kFinNT) void sel_finish_kernel(SelSrc s, SelState* 
   const int nc = in_lds ? int(nall) : 0;
   // overflow with each bucket holding a single key value: those are the keys
   const bool ties = !in_lds && st->cmin[0] == st->cmax[0] && st->cmin[1] == st->cmax[1];
+  const bool count = in_lds && hi0 > 0 && nc <= kFinCount;
   if (tid == 0) {
     pre[0] = ties ? st->cmin[0] : st->prefix[0];
     pre[1] = ties ? st->cmin[1] : st->prefix[1];
@@ -364,57 +371,108 @@ __global__ __launch_bounds__(kFinNT) void sel_finish_kernel(SelSrc s, SelState* 
   const T med = deviation ? T(st->med) : T(0);
   const int64_t E = (s.mode == 2) ? s.n : s.count;
   __syncthreads();
-  for (int hi = ties ? 0 : hi0; hi > 0;) {
+  if (count) {
+    // rank of key i under its prefix = keys of the same prefix below it;
+    // threads holding equal keys write the same value
+    uint64_t p0 = pre[0], p1 = pre[1];
+    long long k0 = ks[0], k1 = ks[1];
+    __syncthreads();
+    if (tid < nc) {
+      const uint64_t ki = cand[tid], top = ki >> hi0;
+      unsigned less = 0, eq = 0;
+      for (int j = 0; j < nc; ++j) {
+        const uint64_t kj = cand[j];
+        const bool same = (kj >> hi0) == top;
+        less += (same && kj < ki) ? 1u : 0u;
+        eq += (same && kj == ki) ? 1u : 0u;
+      }
+      if (top == p0 && less <= k0 && k0 < (long long)(less + eq)) pre[0] = ki;
+      if (top == p1 && less <= k1 && k1 < (long long)(less + eq)) pre[1] = ki;
+    }
+    __syncthreads();
+  }
+  for (int hi = (ties || count) ? 0 : hi0; hi > 0;) {
     const int lo = hi > kBits ? hi - kBits : 0;
     for (int i = tid; i < 2 * kBins; i += kFinNT) hist[i] = 0;
     __syncthreads();
     const uint64_t p0 = pre[0], p1 = pre[1];
     const bool same = (p0 == p1);
     const uint64_t dmask = (uint64_t(1) << (hi - lo)) - 1;
-    auto count_key = [&](uint64_t k) {
-      const uint64_t top = k >> hi;
-      const unsigned d = unsigned((k >> lo) & dmask);
-      if (top == p0) atomicAdd(&hist[d], 1u);
-      if (!same && top == p1) atomicAdd(&hist[kBins + d], 1u);
-    };
+    constexpr int per = kBins / kFinNT;
     if (in_lds) {
-      for (int i = tid; i < nc; i += kFinNT) count_key(cand[i]);
+      for (int i = tid; i < nc; i += kFinNT) {
+        const uint64_t k = cand[i], top = k >> hi;
+        const unsigned inc = (top == p0 ? 1u : 0u) + (top == p1 ? 0x10000u : 0u);
+        if (inc) atomicAdd(&hist[unsigned((k >> lo) & dmask)], inc);
+      }
+      __syncthreads();
+      unsigned loc = 0;
+#pragma unroll
+      for (int i = 0; i < per; ++i) loc += hist[tid * per + i];
+      const unsigned incl = fin_scan(loc, wsum);
+      const unsigned before = incl - loc;
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int sh = 16 * r;
+        const long long k = ks[r];
+        const long long b = (before >> sh) & 0xffffu, e = (incl >> sh) & 0xffffu;
+        if (b <= k && k < e) {
+          long long acc = b;
+          for (int i = 0; i < per; ++i) {
+            const long long c = (hist[tid * per + i] >> sh) & 0xffffu;
+            if (acc + c > k) {
+              dsel[r] = tid * per + i;
+              below[r] = acc;
+              break;
+            }
+            acc += c;
+          }
+        }
+      }
+      __syncthreads();
+      if (tid < 2) {
+        ks[tid] -= below[tid];
+        pre[tid] = (pre[tid] << (hi - lo)) | uint64_t(dsel[tid]);
+      }
+      __syncthreads();
     } else {
       for (int64_t e = tid; e < E; e += kFinNT) {
         T v;
         if (!sel_value<T>(s, e, v)) continue;
         if (deviation) v = fabs(v - med);
-        count_key(okey(v));
+        const uint64_t k = okey(v), top = k >> hi;
+        const unsigned d = unsigned((k >> lo) & dmask);
+        if (top == p0) atomicAdd(&hist[d], 1u);
+        if (!same && top == p1) atomicAdd(&hist[kBins + d], 1u);
       }
-    }
-    __syncthreads();
-    constexpr int per = kBins / kFinNT;
-    for (int r = 0; r < 2; ++r) {
-      const unsigned* hr = hist + (same ? 0 : r) * kBins;
-      const long long k = ks[r];
-      unsigned loc = 0;
+      __syncthreads();
+      for (int r = 0; r < 2; ++r) {
+        const unsigned* hr = hist + (same ? 0 : r) * kBins;
+        const long long k = ks[r];
+        unsigned loc = 0;
 #pragma unroll
-      for (int i = 0; i < per; ++i) loc += hr[tid * per + i];
-      const unsigned incl = fin_scan(loc, wsum);
-      const long long before = (long long)(incl - loc);
-      if (before <= k && k < (long long)incl) {
-        long long acc = before;
-        for (int i = 0; i < per; ++i) {
-          const unsigned c = hr[tid * per + i];
-          if (acc + c > k) {
-            dsel[r] = tid * per + i;
-            below[r] = acc;
-            break;
+        for (int i = 0; i < per; ++i) loc += hr[tid * per + i];
+        const unsigned incl = fin_scan(loc, wsum);
+        const long long before = (long long)(incl - loc);
+        if (before <= k && k < (long long)incl) {
+          long long acc = before;
+          for (int i = 0; i < per; ++i) {
+            const unsigned c = hr[tid * per + i];
+            if (acc + c > k) {
+              dsel[r] = tid * per + i;
+              below[r] = acc;
+              break;
+            }
+            acc += c;
           }
-          acc += c;
         }
+        __syncthreads();
+        if (tid == 0) {
+          ks[r] = k - below[r];
+          pre[r] = (pre[r] << (hi - lo)) | uint64_t(dsel[r]);
+        }
+        __syncthreads();
       }
-      __syncthreads();
-      if (tid == 0) {
-        ks[r] = k - below[r];
-        pre[r] = (pre[r] << (hi - lo)) | uint64_t(dsel[r]);
-      }
-      __syncthreads();
     }
     hi = lo;
   }
