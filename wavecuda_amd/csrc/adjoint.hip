@@ -99,14 +99,32 @@ int fwd_adj_t(const void* gxmod, void* gx, int64_t n, int64_t count, int J, int 
 }
 
 constexpr int kGradNT = 256;
-constexpr int kGradChunks = 16;  // blocks per (signal, level)
+constexpr int kGradMinPer = 4096;         // elements per workgroup, at least
+constexpr int64_t kGradMaxPartials = 65536;  // partial sums per call, at most
 
+template <typename T>
+__device__ __forceinline__ void grad_one(T cv, T g, T t, int hardness, T* gcp, double& acc) {
+  const T a = fabs(cv);
+  if (hardness == kSOFT) {
+    const bool pass = a > t;
+    if (gcp) *gcp = pass ? g : T(0);
+    if (pass) acc += double(cv < T(0) ? g : -g);
+  } else if (gcp) {
+    *gcp = (a < t) ? T(0) : g;
+  }
+}
+
+// One workgroup per (chunk, level, signal).  MODWT details of a level are
+// contiguous and move as element pairs; the DWT's are strided.  The
+// workgroup's sum is reduced in a fixed order (lanes by shuffles, then the
+// four waves), so results repeat bit for bit.
 template <typename T>
 __global__ __launch_bounds__(kGradNT) void thresh_grad_k(const T* __restrict__ c0,
                                                          const T* __restrict__ gy0,
                                                          T* __restrict__ gc0,
                                                          double* __restrict__ partial, int64_t n,
                                                          int J, int ttype, Thr1 th, int write_gc) {
+  using V = typename Vec2<T>::type;
   const int l = th.minlevel + int(blockIdx.y);
   const int nl = int(gridDim.y);
   const int64_t sig = blockIdx.z;
@@ -114,42 +132,43 @@ __global__ __launch_bounds__(kGradNT) void thresh_grad_k(const T* __restrict__ c
   const T* __restrict__ c = c0 + sig * per;
   const T* __restrict__ gy = gy0 + sig * per;
   T* __restrict__ gc = write_gc ? gc0 + sig * per : nullptr;
-  int64_t off, stride, cnt;
-  if (ttype == kDWT) {
-    off = int64_t(1) << l;
-    stride = int64_t(2) << l;
-    cnt = n >> (l + 1);
-  } else {
-    off = 2 * n * l + n;
-    stride = 1;
-    cnt = n;
-  }
-  const int64_t chunk = (cnt + gridDim.x - 1) / gridDim.x;
-  const int64_t e0 = int64_t(blockIdx.x) * chunk;
-  const int64_t e1 = std::min<int64_t>(cnt, e0 + chunk);
   const T t = th.levels ? T(th.levels[l]) : T(th.value);
   double acc = 0.0;
-  for (int64_t e = e0 + threadIdx.x; e < e1; e += kGradNT) {
-    const int64_t i = off + e * stride;
-    const T cv = c[i], g = gy[i];
-    const T a = fabs(cv);
-    if (th.hardness == kSOFT) {
-      const bool pass = a > t;
-      if (write_gc) gc[i] = pass ? g : T(0);
-      if (pass) acc += double(cv < T(0) ? g : -g);
-    } else if (write_gc) {
-      gc[i] = (a < t) ? T(0) : g;
+  if (ttype == kDWT) {
+    const int64_t off = int64_t(1) << l, stride = int64_t(2) << l, cnt = n >> (l + 1);
+    const int64_t chunk = (cnt + gridDim.x - 1) / gridDim.x;
+    const int64_t e0 = int64_t(blockIdx.x) * chunk, e1 = std::min<int64_t>(cnt, e0 + chunk);
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += kGradNT) {
+      const int64_t i = off + e * stride;
+      grad_one(c[i], gy[i], t, th.hardness, gc ? gc + i : nullptr, acc);
+    }
+  } else {
+    const int64_t off = 2 * n * l + n, cnt = n / 2;  // element pairs of D_{l+1}
+    const int64_t chunk = (cnt + gridDim.x - 1) / gridDim.x;
+    const int64_t e0 = int64_t(blockIdx.x) * chunk, e1 = std::min<int64_t>(cnt, e0 + chunk);
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += kGradNT) {
+      const int64_t i = off + 2 * e;
+      const V cv = *reinterpret_cast<const V*>(c + i);
+      const V g = *reinterpret_cast<const V*>(gy + i);
+      if (gc) {
+        V o;
+        grad_one(cv.x, g.x, t, th.hardness, &o.x, acc);
+        grad_one(cv.y, g.y, t, th.hardness, &o.y, acc);
+        *reinterpret_cast<V*>(gc + i) = o;
+      } else {
+        grad_one(cv.x, g.x, t, th.hardness, static_cast<T*>(nullptr), acc);
+        grad_one(cv.y, g.y, t, th.hardness, static_cast<T*>(nullptr), acc);
+      }
     }
   }
-  __shared__ double red[kGradNT];
-  red[threadIdx.x] = acc;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  __shared__ double red[kGradNT / 64];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
-  for (int o = kGradNT / 2; o > 0; o >>= 1) {
-    if (int(threadIdx.x) < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
-  }
   if (threadIdx.x == 0)
-    partial[(sig * nl + (l - th.minlevel)) * int64_t(gridDim.x) + blockIdx.x] = red[0];
+    partial[(sig * nl + (l - th.minlevel)) * int64_t(gridDim.x) + blockIdx.x] =
+        (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // glam[l] = sum over signals and chunks of the level's partials (fixed
@@ -198,13 +217,18 @@ int thresh_grad(const void* c, const void* gy, void* gc, double* glam, int64_t n
     WH_HIP_CHECK(hipMemcpyAsync(gc, gy, size_t(per) * size_t(count) * esz,
                                 hipMemcpyDeviceToDevice, st));
   const int nl = thr->maxlevel - thr->minlevel + 1;
+  // workgroups per (signal, level): at least kGradMinPer elements each, and
+  // at most kGradMaxPartials partial sums in all
+  const int64_t longest = ttype == kDWT ? n / 2 : n;
+  const int64_t nchunk = std::max<int64_t>(
+      1, std::min<int64_t>(longest / kGradMinPer, kGradMaxPartials / (count * nl)));
   void* p = nullptr;
-  const size_t np = size_t(count) * size_t(nl) * kGradChunks;
+  const size_t np = size_t(count) * size_t(nl) * size_t(nchunk);
   int rc = stats_get(np * sizeof(double), &p);
   if (rc) return rc;
   double* partial = static_cast<double*>(p);
   Thr1 th{thr->hardness, thr->minlevel, thr->maxlevel, thr->value, thr->levels};
-  const dim3 grid(kGradChunks, unsigned(nl), unsigned(count));
+  const dim3 grid{unsigned(nchunk), unsigned(nl), unsigned(count)};
   if (dtype == WH_F32)
     hipLaunchKernelGGL(thresh_grad_k<float>, grid, dim3(kGradNT), 0, st,
                        static_cast<const float*>(c), static_cast<const float*>(gy),
@@ -216,7 +240,7 @@ int thresh_grad(const void* c, const void* gy, void* gc, double* glam, int64_t n
   WH_HIP_CHECK(hipGetLastError());
   if (glam) {
     hipLaunchKernelGGL(thresh_grad_reduce_k, dim3(unsigned(J)), dim3(kGradNT), 0, st, partial,
-                       glam, count, kGradChunks, thr->minlevel, thr->maxlevel);
+                       glam, count, int(nchunk), thr->minlevel, thr->maxlevel);
     WH_HIP_CHECK(hipGetLastError());
   }
   return WH_OK;
