@@ -8,6 +8,7 @@ import pytest
 import torch
 
 import oracle_bind as O
+from wavecuda_amd import _lib as W
 from wavecuda_amd import device as D
 from wavecuda_amd import rsurface as R
 
@@ -33,6 +34,33 @@ def test_modwt_labels_match_oracle(n, J):
     np.testing.assert_array_equal(lev.cpu().numpy(), ol)
     np.testing.assert_array_equal(tr.cpu().numpy(), ot)
     np.testing.assert_array_equal(kind.cpu().numpy(), ok)
+
+
+@pytest.mark.parametrize("case", ["unaligned", "no_translate", "n2"])
+def test_modwt_labels_scalar_and_partial(case):
+    """The label columns through the C ABI with the element-wise kernel
+    (columns one element off their 16-byte alignment, or n = 2) and with a
+    column left out (NULL)."""
+    n, J = (2, 1) if case == "n2" else (1 << 12, 6)
+    m = 2 * n * J
+    print(f"modwt labels {case} n={n} J={J}", flush=True)
+    off = 1 if case == "unaligned" else 0
+    lev = torch.full((m + off,), -7, dtype=torch.int32, device="cuda")
+    tr = torch.full((m + off,), -7.0, dtype=torch.float64, device="cuda")
+    kind = torch.full((m + off,), 9, dtype=torch.uint8, device="cuda")
+    trp = None if case == "no_translate" else tr.data_ptr() + 8 * off
+    W.check(W.lib().wh_coeff_labels(n, J, W.MODWT, W.HAAR, lev.data_ptr() + 4 * off, trp,
+                                    kind.data_ptr() + off, None), "wh_coeff_labels")
+    torch.cuda.synchronize()
+    ol, ot, ok = O.coeff_labels(n, J, 1)
+    np.testing.assert_array_equal(lev[off:].cpu().numpy(), ol)
+    np.testing.assert_array_equal(kind[off:].cpu().numpy(), ok)
+    if trp is None:
+        assert (tr == -7.0).all()
+    else:
+        np.testing.assert_array_equal(tr[off:].cpu().numpy(), ot)
+    if off:
+        assert lev[0].item() == -7 and tr[0].item() == -7.0 and kind[0].item() == 9
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
