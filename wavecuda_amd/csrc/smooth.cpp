@@ -7,9 +7,12 @@
 // validation follows the reference's CPU algorithm (src/thresh.cpp:203-342)
 // step for step: the universal threshold of the full transform brackets a
 // golden-section search over the threshold of the even/odd half-length
-// transforms, each candidate costing two fused threshold+inverse passes and
-// one interpolation-error reduction; the search's scalar bookkeeping runs
-// on the host with the reference's exact arithmetic.
+// transforms, each candidate costing one fused threshold+inverse launch over
+// both halves and one interpolation-error reduction.  The search's scalar
+// steps run on the device with the reference's arithmetic (cv_step), so
+// candidates follow one another without the host; the host reads the
+// search state back only to learn how many more candidates the loop is
+// certain to evaluate.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -28,6 +31,8 @@ struct StatsBuf {
   double partial[kMaxPartials];
   double vals[16];
   double levels[64];
+  double cvlev[64];  // the candidate threshold of every level
+  CvState cv;
 };
 
 int stats_buf(StatsBuf** out) {
@@ -50,6 +55,25 @@ int inv(int ttype, const void* w, void* x, int64_t n, int J, int filt, int dtype
         const wh_thresh* thr, hipStream_t st) {
   return ttype == kDWT ? dwt_inverse(w, x, n, J, filt, dtype, thr, st)
                        : modwt_inverse(w, x, n, 1, J, filt, dtype, thr, st);
+}
+
+// Candidates the search will certainly evaluate from state `cs` on (not
+// stopped): the pending one, then one more for every loop test that must
+// pass -- the bracket narrows by R per step and t1 + t2 <= 2 t3, so
+// |t3 - t0| R^j > 2 tol t3 (with a margin for rounding) keeps the loop going,
+// as long as the iteration cap allows.
+int cv_sure_evals(const CvState& cs, double tol) {
+  const double R = 0.61803399, margin = 1. - 1e-6;
+  double w = std::fabs(cs.t3 - cs.t0) * margin;
+  const double lim = tol * 2. * cs.t3;
+  int it = cs.iter + (cs.nevals >= 2 ? 1 : 0);
+  int k = 1;
+  while (k < 64 && it <= 50 && w > lim) {
+    ++k;
+    ++it;
+    w *= R * margin;
+  }
+  return k;
 }
 
 int d2h(double* host, const double* dev, hipStream_t st) {
@@ -160,8 +184,6 @@ int smooth(void* x, int64_t n, int J, int ttype, int filt, int dtype, int thresh
 
   if ((rc = fwd(ttype, x, W, n, J, filt, dtype, st))) return rc;
   if ((rc = univ_thresh_dev(W, n, J, ttype, dtype, minlevel, maxlevel, &sb->vals[0], st))) return rc;
-  double tc = 0.0;
-  if ((rc = d2h(&tc, &sb->vals[0], st))) return rc;
   if ((rc = split_halves(x, xe, xo, n, dtype, st))) return rc;
   // the two half-length MODWTs run as one two-signal launch (xe, xo and
   // We, Wo and ye, yo are adjacent): twice the workgroups, half the launches
@@ -172,13 +194,14 @@ int smooth(void* x, int64_t n, int J, int ttype, int filt, int dtype, int thresh
     return rc;
   }
 
-  auto eval = [&](double t, double* m) -> int {
-    wh_thresh th{};
-    th.enabled = 1;
-    th.hardness = hardness;
-    th.minlevel = minlevel;
-    th.maxlevel = maxlevel - 1;
-    th.value = t;
+  // one candidate: the halves thresholded at the point cv_step last wrote
+  wh_thresh th{};
+  th.enabled = 1;
+  th.hardness = hardness;
+  th.minlevel = minlevel;
+  th.maxlevel = maxlevel - 1;
+  th.levels = sb->cvlev;
+  auto eval = [&]() -> int {
     int r;
     if (ttype == kDWT) {
       if ((r = inv(ttype, We, ye, lenh, Jh, filt, dtype, &th, st))) return r;
@@ -187,42 +210,28 @@ int smooth(void* x, int64_t n, int J, int ttype, int filt, int dtype, int thresh
       return r;
     }
     if ((r = interp_mse(x, ye, yo, n, dtype, sb->partial, &sb->vals[1], st))) return r;
-    return d2h(m, &sb->vals[1], st);
+    return cv_step(&sb->cv, &sb->vals[1], sb->cvlev, Jh, tol, st);
   };
-
-  const double R = 0.61803399, C = 1. - R;
-  double tb = tc / 2.;
-  double t0 = 0., t2 = tb, t1 = tb - C * (tb - 0.), t3 = tc;
-  double m1 = 0, m2 = 0;
-  if ((rc = eval(t1, &m1))) return rc;
-  if ((rc = eval(t2, &m2))) return rc;
-  unsigned iter = 0;
-  while (std::fabs(t3 - t0) > tol * (t1 + t2)) {
-    if (iter > 50) break;
-    if (m2 < m1) {
-      t0 = t1;
-      t1 = t2;
-      t2 = R * t2 + C * t3;
-      m1 = m2;
-      if ((rc = eval(t2, &m2))) return rc;
-    } else {
-      t3 = t2;
-      t2 = t1;
-      t1 = R * t1 + C * t0;
-      m2 = m1;
-      if ((rc = eval(t1, &m1))) return rc;
-    }
-    iter++;
+  if ((rc = cv_init(&sb->cv, &sb->vals[0], sb->cvlev, Jh, st))) return rc;
+  CvState cs{};
+  for (int batch = 2;;) {  // t1 and t2 first
+    for (int i = 0; i < batch; ++i)
+      if ((rc = eval())) return rc;
+    WH_HIP_CHECK(hipMemcpyAsync(&cs, &sb->cv, sizeof(CvState), hipMemcpyDeviceToHost, st));
+    WH_HIP_CHECK(hipStreamSynchronize(st));
+    if (cs.stop) break;
+    batch = cv_sure_evals(cs, tol);
   }
+  const double t1 = cs.t1, t2 = cs.t2, m1 = cs.m1, m2 = cs.m2;
   double t = m1 < m2 ? t1 : t2;
   t = t / (std::sqrt(1. - std::log(2.) / std::log(double(n))));
-  wh_thresh th{};
-  th.enabled = 1;
-  th.hardness = hardness;
-  th.minlevel = minlevel;
-  th.maxlevel = maxlevel;
-  th.value = t;
-  if ((rc = inv(ttype, W, x, n, J, filt, dtype, &th, st))) return rc;
+  wh_thresh tf{};
+  tf.enabled = 1;
+  tf.hardness = hardness;
+  tf.minlevel = minlevel;
+  tf.maxlevel = maxlevel;
+  tf.value = t;
+  if ((rc = inv(ttype, W, x, n, J, filt, dtype, &tf, st))) return rc;
   WH_HIP_CHECK(hipStreamSynchronize(st));
   if (thresh) *thresh = t;
   return WH_OK;

@@ -653,4 +653,75 @@ int broadcast_threshold(double* dst, const double* src, double scale, int J, hip
   return WH_OK;
 }
 
+// ------------------------------------------------- cross-validation search
+// One thread each: the scalar steps of the reference's golden-section loop
+// (src/thresh.cpp:296-330), with products and sums rounded separately as
+// in its host build, so the visited thresholds are the same doubles.
+__global__ void cv_init_k(CvState* cs, const double* tc, double* levels, int J) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0) return;
+  const double C = 1. - 0.61803399;
+  const double t3 = *tc, tb = t3 / 2.;
+  cs->t0 = 0.;
+  cs->t2 = tb;
+  cs->t1 = tb - C * (tb - 0.);
+  cs->t3 = t3;
+  cs->m1 = cs->m2 = 0.;
+  cs->iter = 0;
+  cs->pending = 1;
+  cs->nevals = 0;
+  cs->stop = 0;
+  for (int l = 0; l < J; ++l) levels[l] = cs->t1;
+}
+
+__global__ void cv_step_k(CvState* cs, const double* m, double* levels, int J, double tol) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0 || cs->stop) return;
+  const double R = 0.61803399, C = 1. - R;
+  if (cs->pending == 1)
+    cs->m1 = *m;
+  else
+    cs->m2 = *m;
+  const int ne = ++cs->nevals;
+  double next;
+  if (ne == 1) {  // the first two evaluations: t1, then t2
+    cs->pending = 2;
+    next = cs->t2;
+  } else {
+    if (ne > 2) cs->iter++;  // the loop body that ordered this evaluation ends
+    if (!(fabs(cs->t3 - cs->t0) > tol * (cs->t1 + cs->t2)) || cs->iter > 50) {
+      cs->stop = 1;
+      return;
+    }
+    if (cs->m2 < cs->m1) {
+      cs->t0 = cs->t1;
+      cs->t1 = cs->t2;
+      cs->t2 = R * cs->t2 + C * cs->t3;
+      cs->m1 = cs->m2;
+      cs->pending = 2;
+      next = cs->t2;
+    } else {
+      cs->t3 = cs->t2;
+      cs->t2 = cs->t1;
+      cs->t1 = R * cs->t1 + C * cs->t0;
+      cs->m2 = cs->m1;
+      cs->pending = 1;
+      next = cs->t1;
+    }
+  }
+  for (int l = 0; l < J; ++l) levels[l] = next;
+}
+
+int cv_init(CvState* cs, const double* tc, double* levels, int J, hipStream_t st) {
+  hipLaunchKernelGGL(cv_init_k, dim3(1), dim3(64), 0, st, cs, tc, levels, J);
+  WH_HIP_CHECK(hipGetLastError());
+  return WH_OK;
+}
+
+int cv_step(CvState* cs, const double* m, double* levels, int J, double tol, hipStream_t st) {
+  hipLaunchKernelGGL(cv_step_k, dim3(1), dim3(64), 0, st, cs, m, levels, J, tol);
+  WH_HIP_CHECK(hipGetLastError());
+  return WH_OK;
+}
+
 }  // namespace wh

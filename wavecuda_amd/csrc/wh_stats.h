@@ -46,6 +46,15 @@ struct SelState {
 
 constexpr int kMaxPartials = 1024;
 
+// Golden-section search of the cross-validated threshold, kept on the device
+// (reference src/thresh.cpp:296-330): the bracket t0 <= t1 <= t2 <= t3, the
+// errors m1, m2 at t1, t2, the loop count, which point the pending
+// evaluation is for (1: t1, 2: t2), evaluations consumed so far, and stop.
+struct CvState {
+  double t0, t1, t2, t3, m1, m2;
+  int iter, pending, nevals, stop;
+};
+
 // in-place threshold of `count` transformed signals (DWT: n apart, MODWT:
 // 2nJ apart)
 int threshold_inplace(void* c, int64_t n, int64_t count, int J, int ttype, int dtype,
@@ -58,5 +67,10 @@ int interp_mse(const void* xn, const void* ye, const void* yo, int64_t len, int 
                double* partial_buf, double* out, hipStream_t st);
 int split_halves(const void* x, void* e, void* o, int64_t len, int dtype, hipStream_t st);
 int broadcast_threshold(double* dst, const double* src, double scale, int J, hipStream_t st);
+// bracket [0, *tc] and the first point t1 into levels[0..J)
+int cv_init(CvState* cs, const double* tc, double* levels, int J, hipStream_t st);
+// take the error *m of the pending point, advance the search, and write the
+// next point into levels[0..J) (nothing once stopped)
+int cv_step(CvState* cs, const double* m, double* levels, int J, double tol, hipStream_t st);
 
 }  // namespace wh
