@@ -9,7 +9,7 @@
 // golden-section search over the threshold of the even/odd half-length
 // transforms, each candidate costing one fused threshold+inverse launch over
 // both halves and one interpolation-error reduction.  The search's scalar
-// steps run on the device with the reference's arithmetic (cv_step), so
+// steps run on the device with the reference's arithmetic (cv_advance), so
 // candidates follow one another without the host; the host reads the
 // search state back only to learn how many more candidates the loop is
 // certain to evaluate.
@@ -194,13 +194,18 @@ int smooth(void* x, int64_t n, int J, int ttype, int filt, int dtype, int thresh
     return rc;
   }
 
-  // one candidate: the halves thresholded at the point cv_step last wrote
+  // one candidate: the halves thresholded at the point cv_advance last wrote
   wh_thresh th{};
   th.enabled = 1;
   th.hardness = hardness;
   th.minlevel = minlevel;
   th.maxlevel = maxlevel - 1;
   th.levels = sb->cvlev;
+  CvArgs cva;
+  cva.cs = &sb->cv;
+  cva.levels = sb->cvlev;
+  cva.J = Jh;
+  cva.tol = tol;
   auto eval = [&]() -> int {
     int r;
     if (ttype == kDWT) {
@@ -209,8 +214,8 @@ int smooth(void* x, int64_t n, int J, int ttype, int filt, int dtype, int thresh
     } else if ((r = modwt_inverse(We, ye, lenh, 2, Jh, filt, dtype, &th, st))) {
       return r;
     }
-    if ((r = interp_mse(x, ye, yo, n, dtype, sb->partial, &sb->vals[1], st))) return r;
-    return cv_step(&sb->cv, &sb->vals[1], sb->cvlev, Jh, tol, st);
+    // the error reduction's last block also advances the search
+    return interp_mse(x, ye, yo, n, dtype, sb->partial, &sb->vals[1], st, cva);
   };
   if ((rc = cv_init(&sb->cv, &sb->vals[0], sb->cvlev, Jh, st))) return rc;
   CvState cs{};

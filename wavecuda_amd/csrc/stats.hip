@@ -547,6 +547,74 @@ int select_mad(const SelSrc& s, int dtype, SelState* st, double factor, double* 
   return select_median(s, dtype, true, st, factor, out, stream);
 }
 
+// ------------------------------------------------- cross-validation search
+// The scalar steps of the reference's golden-section loop
+// (src/thresh.cpp:296-330), with products and sums rounded separately as
+// in its host build, so the visited thresholds are the same doubles.  One
+// thread: take the error m of the pending point, advance the bracket and
+// write the next point into levels[0..J).
+__device__ __noinline__ void cv_advance(CvState* cs, double m, double* levels, int J,
+                                        double tol) {
+#pragma clang fp contract(off)
+  if (cs->stop) return;
+  const double R = 0.61803399, C = 1. - R;
+  if (cs->pending == 1)
+    cs->m1 = m;
+  else
+    cs->m2 = m;
+  const int ne = ++cs->nevals;
+  double next;
+  if (ne == 1) {  // the first two evaluations: t1, then t2
+    cs->pending = 2;
+    next = cs->t2;
+  } else {
+    if (ne > 2) cs->iter++;  // the loop body that ordered this evaluation ends
+    if (!(fabs(cs->t3 - cs->t0) > tol * (cs->t1 + cs->t2)) || cs->iter > 50) {
+      cs->stop = 1;
+      return;
+    }
+    if (cs->m2 < cs->m1) {
+      cs->t0 = cs->t1;
+      cs->t1 = cs->t2;
+      cs->t2 = R * cs->t2 + C * cs->t3;
+      cs->m1 = cs->m2;
+      cs->pending = 2;
+      next = cs->t2;
+    } else {
+      cs->t3 = cs->t2;
+      cs->t2 = cs->t1;
+      cs->t1 = R * cs->t1 + C * cs->t0;
+      cs->m2 = cs->m1;
+      cs->pending = 1;
+      next = cs->t1;
+    }
+  }
+  for (int l = 0; l < J; ++l) levels[l] = next;
+}
+
+__global__ void cv_init_k(CvState* cs, const double* tc, double* levels, int J) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0) return;
+  const double C = 1. - 0.61803399;
+  const double t3 = *tc, tb = t3 / 2.;
+  cs->t0 = 0.;
+  cs->t2 = tb;
+  cs->t1 = tb - C * (tb - 0.);
+  cs->t3 = t3;
+  cs->m1 = cs->m2 = 0.;
+  cs->iter = 0;
+  cs->pending = 1;
+  cs->nevals = 0;
+  cs->stop = 0;
+  for (int l = 0; l < J; ++l) levels[l] = cs->t1;
+}
+
+int cv_init(CvState* cs, const double* tc, double* levels, int J, hipStream_t st) {
+  hipLaunchKernelGGL(cv_init_k, dim3(1), dim3(64), 0, st, cs, tc, levels, J);
+  WH_HIP_CHECK(hipGetLastError());
+  return WH_OK;
+}
+
 // --------------------------------------------------------- interp_mse
 template <typename T>
 __global__ __launch_bounds__(256) void interp_mse_partial(const T* __restrict__ xn,
@@ -584,7 +652,7 @@ __global__ __launch_bounds__(256) void interp_mse_partial(const T* __restrict__ 
 }
 
 __global__ __launch_bounds__(256) void sum_partials(const double* partial, int np, double scale,
-                                                    double* out) {
+                                                    double* out, CvArgs cv) {
   __shared__ double red[256];
   double acc = 0.0;
   for (int i = threadIdx.x; i < np; i += 256) acc += partial[i];
@@ -594,13 +662,17 @@ __global__ __launch_bounds__(256) void sum_partials(const double* partial, int n
     if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = red[0] * scale;
+  if (threadIdx.x == 0) {
+    const double v = red[0] * scale;
+    *out = v;
+    if (cv.cs) cv_advance(cv.cs, v, cv.levels, cv.J, cv.tol);
+  }
 }
 
 int interp_mse(const void* xn, const void* ye, const void* yo, int64_t len, int dtype,
-               double* partial_buf, double* out, hipStream_t st) {
+               double* partial_buf, double* out, hipStream_t st, const CvArgs& cv) {
   const int64_t lenh = len / 2;
-  const int64_t per_block = 2048;
+  const int64_t per_block = 512;
   const int np = int(std::min<int64_t>((lenh + per_block - 1) / per_block, kMaxPartials));
   const int64_t pb = (lenh + np - 1) / np;
   if (dtype == WH_F32)
@@ -611,7 +683,7 @@ int interp_mse(const void* xn, const void* ye, const void* yo, int64_t len, int 
     hipLaunchKernelGGL(interp_mse_partial<double>, dim3(np), dim3(256), 0, st,
                        static_cast<const double*>(xn), static_cast<const double*>(ye),
                        static_cast<const double*>(yo), len, pb, partial_buf);
-  hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, st, partial_buf, np, 0.5, out);
+  hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, st, partial_buf, np, 0.5, out, cv);
   WH_HIP_CHECK(hipGetLastError());
   return WH_OK;
 }
@@ -649,77 +721,6 @@ __global__ void fill_levels(double* dst, const double* src, double scale, int J)
 
 int broadcast_threshold(double* dst, const double* src, double scale, int J, hipStream_t st) {
   hipLaunchKernelGGL(fill_levels, dim3(1), dim3(64), 0, st, dst, src, scale, J);
-  WH_HIP_CHECK(hipGetLastError());
-  return WH_OK;
-}
-
-// ------------------------------------------------- cross-validation search
-// One thread each: the scalar steps of the reference's golden-section loop
-// (src/thresh.cpp:296-330), with products and sums rounded separately as
-// in its host build, so the visited thresholds are the same doubles.
-__global__ void cv_init_k(CvState* cs, const double* tc, double* levels, int J) {
-#pragma clang fp contract(off)
-  if (threadIdx.x != 0) return;
-  const double C = 1. - 0.61803399;
-  const double t3 = *tc, tb = t3 / 2.;
-  cs->t0 = 0.;
-  cs->t2 = tb;
-  cs->t1 = tb - C * (tb - 0.);
-  cs->t3 = t3;
-  cs->m1 = cs->m2 = 0.;
-  cs->iter = 0;
-  cs->pending = 1;
-  cs->nevals = 0;
-  cs->stop = 0;
-  for (int l = 0; l < J; ++l) levels[l] = cs->t1;
-}
-
-__global__ void cv_step_k(CvState* cs, const double* m, double* levels, int J, double tol) {
-#pragma clang fp contract(off)
-  if (threadIdx.x != 0 || cs->stop) return;
-  const double R = 0.61803399, C = 1. - R;
-  if (cs->pending == 1)
-    cs->m1 = *m;
-  else
-    cs->m2 = *m;
-  const int ne = ++cs->nevals;
-  double next;
-  if (ne == 1) {  // the first two evaluations: t1, then t2
-    cs->pending = 2;
-    next = cs->t2;
-  } else {
-    if (ne > 2) cs->iter++;  // the loop body that ordered this evaluation ends
-    if (!(fabs(cs->t3 - cs->t0) > tol * (cs->t1 + cs->t2)) || cs->iter > 50) {
-      cs->stop = 1;
-      return;
-    }
-    if (cs->m2 < cs->m1) {
-      cs->t0 = cs->t1;
-      cs->t1 = cs->t2;
-      cs->t2 = R * cs->t2 + C * cs->t3;
-      cs->m1 = cs->m2;
-      cs->pending = 2;
-      next = cs->t2;
-    } else {
-      cs->t3 = cs->t2;
-      cs->t2 = cs->t1;
-      cs->t1 = R * cs->t1 + C * cs->t0;
-      cs->m2 = cs->m1;
-      cs->pending = 1;
-      next = cs->t1;
-    }
-  }
-  for (int l = 0; l < J; ++l) levels[l] = next;
-}
-
-int cv_init(CvState* cs, const double* tc, double* levels, int J, hipStream_t st) {
-  hipLaunchKernelGGL(cv_init_k, dim3(1), dim3(64), 0, st, cs, tc, levels, J);
-  WH_HIP_CHECK(hipGetLastError());
-  return WH_OK;
-}
-
-int cv_step(CvState* cs, const double* m, double* levels, int J, double tol, hipStream_t st) {
-  hipLaunchKernelGGL(cv_step_k, dim3(1), dim3(64), 0, st, cs, m, levels, J, tol);
   WH_HIP_CHECK(hipGetLastError());
   return WH_OK;
 }

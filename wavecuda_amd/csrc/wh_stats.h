@@ -63,14 +63,19 @@ int select_median(const SelSrc& s, int dtype, bool deviation, SelState* st, doub
                   double* out, hipStream_t stream);
 int select_mad(const SelSrc& s, int dtype, SelState* st, double factor, double* out,
                hipStream_t stream);
+// search state the CV error reduction advances (cs == nullptr: none)
+struct CvArgs {
+  CvState* cs = nullptr;
+  double* levels = nullptr;
+  int J = 0;
+  double tol = 0.0;
+};
 int interp_mse(const void* xn, const void* ye, const void* yo, int64_t len, int dtype,
-               double* partial_buf, double* out, hipStream_t st);
+               double* partial_buf, double* out, hipStream_t st, const CvArgs& cv = CvArgs{});
 int split_halves(const void* x, void* e, void* o, int64_t len, int dtype, hipStream_t st);
 int broadcast_threshold(double* dst, const double* src, double scale, int J, hipStream_t st);
 // bracket [0, *tc] and the first point t1 into levels[0..J)
 int cv_init(CvState* cs, const double* tc, double* levels, int J, hipStream_t st);
-// take the error *m of the pending point, advance the search, and write the
-// next point into levels[0..J) (nothing once stopped)
-int cv_step(CvState* cs, const double* m, double* levels, int J, double tol, hipStream_t st);
+
 
 }  // namespace wh
