@@ -51,28 +51,39 @@ using KnobKey = std::tuple<int, int64_t, int, int, int>;  // inverse, n, J, filt
 std::mutex g_knob_mu;
 std::map<KnobKey, PlanKnobs> g_knobs;
 
-// Default plan knobs.  An inverse launch of up to 2^18 samples (length
-// times signals) has too few segments for the whole GPU with one
-// 512-thread workgroup per CU; a
-// 256-thread first group (two per CU, two segments per CU) of at most six
-// levels measured faster than every 512-thread plan there (LA8: 0.025 vs
-// 0.028 ms at n = 2^16, 0.032 vs 0.035 at 2^17, 0.037 vs 0.041 at 2^18);
-// from 2^20 on its deeper groups plan worse and the 512-thread default is
-// faster.
-PlanKnobs default_knobs(bool inverse, int64_t n, int64_t count) {
+// Default plan knobs, measured on MI355X (LA8, J = 10; forward and inverse
+// timed alternately, as in a pipeline):
+//  * an inverse launch of up to 2^18 samples (length times signals) has too
+//    few segments for the whole GPU with one 512-thread workgroup per CU: a
+//    256-thread first group (two segments per CU) of at most six levels
+//    (0.025 vs 0.028 ms at n = 2^16, 0.032 vs 0.035 at 2^17);
+//  * a single-signal FP32 inverse of more than 2^20 samples: a 256-thread
+//    first group of at most six levels, four segments per CU (half the FP64
+//    LDS, so several workgroups share a CU): n = 2^22 0.090 -> 0.083 ms,
+//    2^24 0.307 -> 0.258 ms (at 2^20 it measured 2-4 % slower).
+// Rules of the same kind for FP64 at 2^20-2^22 and for the forward, taken
+// from the tuner's isolated timings, were slower in the pipeline (there
+// the 2nJ coefficients fit the last-level cache, and what the other
+// transform leaves in it matters as much as the plan), so those keep the
+// plain defaults.
+PlanKnobs default_knobs(bool inverse, int64_t n, int64_t count, size_t esz) {
   PlanKnobs k;
-  if (inverse && n * count <= (int64_t(1) << 18)) {
+  auto small_inverse = [&k](int per_cu) {
     k.first_nt = kNT1 / 2;
-    k.first_per_cu = 2;
+    k.first_per_cu = per_cu;
     k.g1max = 6;
-  }
+  };
+  if (inverse && n * count <= (int64_t(1) << 18))
+    small_inverse(2);
+  else if (inverse && count == 1 && esz == 4 && n > (int64_t(1) << 20))
+    small_inverse(4);
   return k;
 }
 
 PlanKnobs knobs_for(bool inverse, int64_t n, int J, int filt, size_t esz, int64_t count = 1) {
   std::lock_guard<std::mutex> lk(g_knob_mu);
   auto it = g_knobs.find(KnobKey{inverse ? 1 : 0, n, J, filt, int(esz)});
-  return it == g_knobs.end() ? default_knobs(inverse, n, count) : it->second;
+  return it == g_knobs.end() ? default_knobs(inverse, n, count, esz) : it->second;
 }
 
 // LDS elements of one residue's buffers (the first inverse group keeps the
@@ -367,7 +378,7 @@ int tune_t(int64_t n, int J, int filt, hipStream_t st) {
     constexpr float kMargin = 0.98f;
     const char* env = std::getenv("WH_TUNE_LOG");  // candidate timings to stderr
     const bool log = env && env[0] == '1';
-    std::vector<PlanKnobs> order{default_knobs(inverse, n, 1)};
+    std::vector<PlanKnobs> order{default_knobs(inverse, n, 1, esz)};
     for (auto& c : cands) order.push_back(c.second);
     auto time_knobs = [&](const PlanKnobs& k, float* ms) -> int {
       {
