@@ -152,6 +152,24 @@ def test_smooth_parity(case):
     assert O.relerr(got.cpu().numpy(), want) <= 1e-12
 
 
+@pytest.mark.parametrize("tol", [0.0, 1e-6, 0.5])
+@pytest.mark.parametrize("tt", [0, 1])
+def test_cv_search_tolerances(tt, tol):
+    """The device-side golden-section search against the oracle's host loop
+    where the loop runs to its 51-iteration cap (tol 0), runs long (1e-6)
+    and stops early (0.5); a second call repeats the threshold exactly."""
+    n = 4096
+    nl = 0 if tt == 0 else 8
+    x, _ = noisy_doppler(n, 3.0, seed=23)
+    want, t_want, rc = O.smooth(x, nl, tt, 4, 2, None, 0, 0, 7, tol)
+    assert rc == 0
+    got, t_got = D.smooth(_dev(x), nl, tt, 4, 2, None, 0, 0, 7, tol)
+    assert same_t(t_got, t_want)
+    assert O.relerr(got.cpu().numpy(), want) <= 1e-12
+    _, t_again = D.smooth(_dev(x), nl, tt, 4, 2, None, 0, 0, 7, tol)
+    assert t_again == t_got
+
+
 # GPUSmooth configuration: MODWT J=10, Haar and LA8, levels 1..10, soft and
 # hard, CV and universal, n = 2^18, SNR 5 and 2
 @pytest.mark.parametrize("filt", [1, 4])
