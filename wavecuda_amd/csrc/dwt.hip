@@ -208,12 +208,22 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
     const int64_t Bl = base >> l;
     const int jg = j0 + l;  // global 1-based level
     const T t = thr_level<T>(a.thr, jg - 1);
-    for (int i = tid; i < cnt; i += kNT) {
-      int64_t m = (Bl - el + i) & (Nl - 1);
-      const int64_t pos = (2 * m + 1) << (jg - 1);
-      T d = staged ? pool[pos - (base - HY)] : y[pos];
-      if (t >= T(0)) d = thresh_coef(d, t, a.thr.hardness);
-      dbuf[doff + i] = d;
+    // two loops, so the staged one reads LDS with LDS instructions (one
+    // loop with a select between the two sources compiles to flat loads)
+    if (staged) {
+      for (int i = tid; i < cnt; i += kNT) {
+        const int64_t m = (Bl - el + i) & (Nl - 1);
+        T d = pool[((2 * m + 1) << (jg - 1)) - (base - HY)];
+        if (t >= T(0)) d = thresh_coef(d, t, a.thr.hardness);
+        dbuf[doff + i] = d;
+      }
+    } else {
+      for (int i = tid; i < cnt; i += kNT) {
+        const int64_t m = (Bl - el + i) & (Nl - 1);
+        T d = y[(2 * m + 1) << (jg - 1)];
+        if (t >= T(0)) d = thresh_coef(d, t, a.thr.hardness);
+        dbuf[doff + i] = d;
+      }
     }
     doff += own + PADS;
   }
