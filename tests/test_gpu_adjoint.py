@@ -245,3 +245,41 @@ def test_threshold_grad_without_gc():
                                        J, W.MODWT, W.LA8, W.WH_F64, C.byref(thr), None),
                 "wh_thresh_grad")
     assert torch.equal(gl[0], gl[1])
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("ttype", [W.MODWT, W.DWT])
+def test_threshold_grad_many_chunks(ttype, dtype):
+    """Per-level threshold gradients of several long signals (each level
+    split over many workgroups) against a direct sum, and gc bit for bit."""
+    n, J, count = 1 << 16, 6, 3
+    per = 2 * n * J if ttype == W.MODWT else n
+    rng = np.random.Generator(np.random.PCG64(31))
+    c_np = rng.standard_normal(per * count).astype(np.float64 if dtype == torch.float64 else np.float32)
+    gy_np = rng.standard_normal(per * count).astype(c_np.dtype)
+    lam_np = np.linspace(0.3, 0.9, J)
+    c, gy = torch.tensor(c_np, device="cuda"), torch.tensor(gy_np, device="cuda")
+    lam = torch.tensor(lam_np, device="cuda")
+    gc = torch.empty_like(c)
+    glam = torch.zeros(J, dtype=torch.float64, device="cuda")
+    thr = W.make_thresh(True, W.SOFT, 0, J - 1, 0.0, lam.data_ptr())
+    W.check(W.lib().wh_thresh_grad(c.data_ptr(), gy.data_ptr(), gc.data_ptr(), glam.data_ptr(),
+                                   n, count, J, ttype, W.LA8,
+                                   W.WH_F64 if dtype == torch.float64 else W.WH_F32,
+                                   C.byref(thr), None), "wh_thresh_grad")
+    want_gc = gy_np.copy()
+    want_gl = np.zeros(J)
+    for s in range(count):
+        base = s * per
+        for l in range(J):
+            if ttype == W.MODWT:
+                idx = base + 2 * n * l + n + np.arange(n)
+            else:
+                idx = base + (1 << l) + (2 << l) * np.arange(n >> (l + 1))
+            cv, g = c_np[idx].astype(np.float64), gy_np[idx].astype(np.float64)
+            t = np.float64(c_np.dtype.type(lam_np[l]))
+            keep = np.abs(cv) > t
+            want_gc[idx] = np.where(keep, gy_np[idx], 0)
+            want_gl[l] += np.sum(np.where(keep, np.where(cv < 0, g, -g), 0.0))
+    np.testing.assert_array_equal(gc.cpu().numpy(), want_gc)
+    np.testing.assert_allclose(glam.cpu().numpy(), want_gl, rtol=1e-12, atol=1e-9)
