@@ -79,6 +79,12 @@ def test_median_mad_candidate_overflow(case):
     g = _dev(x)
     assert D.median(g).item() == O.median(x)
     assert D.mad(g).item() == O.mad(x)
+    # FP32: the median is exact; the MAD's deviations round in FP32
+    x32 = x.astype(np.float32)
+    g32 = _dev(x32, torch.float32)
+    assert D.median(g32).item() == O.median(x32.astype(np.float64))
+    m = O.mad(x32.astype(np.float64))
+    assert abs(D.mad(g32).item() - m) <= 1e-6 * max(m, 1e-30)
 
 
 def test_median_special_values():
@@ -87,6 +93,16 @@ def test_median_special_values():
     assert D.median(g).item() == O.median(x)
     y = np.array([3.0, -2.0, -2.0, 7.0])
     assert D.median(_dev(y)).item() == O.median(y)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 10, 1001, 1 << 18])
+def test_median_f32_sizes(n):
+    rng = np.random.Generator(np.random.PCG64(n + 5))
+    x = (np.round(rng.standard_normal(n) * 64) / 64).astype(np.float32)  # many ties
+    g = torch.tensor(x, device="cuda")
+    assert D.median(g).item() == O.median(x.astype(np.float64))
+    m = O.mad(x.astype(np.float64))
+    assert abs(D.mad(g).item() - m) <= 1e-6 * max(m, 1e-30)
 
 
 def test_median_f32():

@@ -6,9 +6,9 @@
 //    radix selection over order-preserving integer keys of the values:
 //    each grid pass builds an LDS-privatised histogram of one 11-bit digit
 //    of the keys that still match the prefix found so far, and a one-block
-//    scan picks the bucket holding the wanted rank; for FP64 keys the few
-//    keys left after the leading digits are compacted and one workgroup
-//    resolves the remaining digits in LDS.  Both middle order
+//    scan picks the bucket holding the wanted rank; the few keys left after
+//    the leading digits are compacted and one workgroup resolves the
+//    remaining digits in LDS.  Both middle order
 //    statistics are tracked at once; results are bit-exact (a median is an
 //    order statistic; the final 0.5(a+b) and 1.4826 m are the reference's
 //    operations);
@@ -491,20 +491,21 @@ __global__ __launch_bounds__(kFinNT) void sel_finish_kernel(SelSrc s, SelState* 
 }
 
 // Number of 11-bit digits resolved by grid passes before the keys still
-// in play are compacted: FP32 keys (three digits) take every digit on the
-// grid -- compaction would save no pass -- and FP64 keys take two digits up
-// to 2^22 values and three beyond, so that the bucket of a middle rank
-// holds about 2^-10 (2^-21) of the values around it and fits the
-// finishing workgroup's LDS.
+// in play are compacted.  After two digits the bucket of a middle rank
+// holds the values within about 2^-13 (FP32 keys) or 2^-10 (FP64 keys) of
+// it; FP64 keys take a third grid digit beyond 2^22 values, so the bucket
+// (2^-21) still fits the finishing workgroup's LDS.  Beyond 2^31 values
+// every digit is a grid pass (the finishing workgroup counts in 32 bits).
 inline int grid_digits(int key_bits, int64_t E) {
-  if (key_bits == 32 || E >= (int64_t(1) << 31)) return (key_bits + kBits - 1) / kBits;
-  return E <= (int64_t(1) << 22) ? 2 : 3;
+  if (E >= (int64_t(1) << 31)) return (key_bits + kBits - 1) / kBits;
+  return (key_bits == 32 || E <= (int64_t(1) << 22)) ? 2 : 3;
 }
 
-// Grid histogram passes over the leading digits, then (FP64) a compaction
-// of the keys still in play and one workgroup that resolves the remaining
-// digits in LDS: seven or nine launches per FP64 median instead of
-// thirteen, three or four passes over the values instead of six.
+// Grid histogram passes over the leading digits, then a compaction of the
+// keys still in play and one workgroup that resolves the remaining digits
+// in LDS: seven launches per median (nine for FP64 beyond 2^22 values)
+// instead of thirteen (FP64) or eight (FP32), three passes over the values
+// (four) instead of six (three).
 template <typename T>
 int select_median_t(const SelSrc& s, bool deviation, SelState* st, double factor, double* out,
                     hipStream_t stream) {
