@@ -17,6 +17,13 @@
  *    the transforms and thresholds must be aligned to two elements (16
  *    bytes for FP64, 8 for FP32; hipMalloc gives 256); WH_ERR_ARG otherwise.
  *
+ *    Scratch: the library keeps per-device work buffers, grown on demand
+ *    and held until exit (the MODWT inverse's level hand-off, DWT staging,
+ *    the selection and cross-validation state, smoothing workspaces).
+ *    Calls on one device that use them must not overlap in time: issue
+ *    them on one stream, or synchronize between streams.  Batch plans own
+ *    their workspace.  Error messages are per thread.
+ *
  * Codes are the reference's: sense FWD=1/BWD=0, ttype DWT=0/MODWT=1
  * (time ordered), filter Haar=1/D4=2/C6=3/LA8=4, hardness HARD=0/SOFT=1,
  * threshtype MANUAL=0/UNIV=1/CV=2, levels 0-based for thresholding.
