@@ -81,16 +81,51 @@ int wh_resolve_levels(int64_t n, int nlevels, int filter);
 
 /* ---- reference .C entry points (host pointers, synchronous) ----------- */
 
-/* reference src/rfunctions.cuh:22 / src/rfunctions.cu:21-57; ttype 2 is
-   the packet-ordered MODWT of the reference's CPU dispatch
-   (src/transform.cpp:39-55), here for all four filters. */
+/* The five symbols the reference package binds (NAMESPACE:26-30,
+   src/rfunctions.cuh:17-35), same argument lists and codes.  All five run
+   on the GPU: the "cpu" names exist so that the package's CPU functions
+   (CPUTransform, CPUReconstruct, CPUThreshold, CPUSmooth) find their
+   symbols; they compute what the reference's CPU path computes (the
+   parity target of every kernel here).  Host arrays move through a pinned
+   staging ring and device buffers that persist across calls.  One call at
+   a time per device (the calls serialise on a per-device lock).
+
+   RcpuTransform / RgpuTransform: reference src/rfunctions.cu:3-57.  DWT:
+   x transformed in place.  MODWT (ttype 1): FWD x -> xmod (2 len nlevels
+   values), BWD xmod -> x.  ttype 2 is the packet-ordered MODWT of the
+   reference's CPU dispatch (src/transform.cpp:39-55), here for all four
+   filters.  A BWD MODWT leaves xmod as it was (the reference's CPU
+   routine also rewrites its S_j slots, j < nlevels, with intermediate
+   reconstructions that the R functions discard: they return x). */
+void RcpuTransform(double* x, double* xmod, int* len, int* sense, int* nlevels, int* ttype,
+                   int* filter, int* filterlen);
 void RgpuTransform(double* x, double* xmod, int* len, int* sense, int* nlevels, int* ttype,
                    int* filter, int* filterlen);
-/* reference src/rfunctions.cuh:34 / src/rfunctions.cu:102-136; all four
-   filters, both transform types and all three threshold types. */
+/* RcpuThreshold: reference src/rfunctions.cu:59-69 (src/thresh.cpp:113-140).
+   The details of the 0-based levels minlevel..maxlevel of x (DWT) or xmod
+   (MODWT, both orders) thresholded in place at *thresh. */
+void RcpuThreshold(double* x, double* xmod, int* len, int* nlevels, int* ttype, int* filter,
+                   int* filterlen, double* thresh, int* hardness, int* minlevel, int* maxlevel);
+/* RcpuSmooth / RgpuSmooth: reference src/rfunctions.cu:71-136.  x smoothed
+   in place; threshtype MANUAL (*thresh is the value), UNIV or CV (*thresh
+   receives the threshold chosen); all four filters, both transforms (the
+   reference's RgpuSmooth offers MANUAL and CV only). */
+void RcpuSmooth(double* x, int* len, int* nlevels, int* ttype, int* filter, int* filterlen,
+                int* threshtype, double* thresh, int* hardness, int* minlevel, int* maxlevel,
+                double* tol);
 void RgpuSmooth(double* x, int* len, int* nlevels, int* ttype, int* filter, int* filterlen,
                 int* threshtype, double* thresh, int* hardness, int* minlevel, int* maxlevel,
                 double* tol);
+/* RgpuSmoothList: the `.C` form of the batched smoother (wh_batch_* below),
+   in place of the reference's list stub RgpuTransformList
+   (src/rfunctions.cu:139-154; R/rfunctions.R:614-643 commented).  x holds
+   *count signals back to back (signal i: lens[i] samples); each is smoothed
+   in place with its own filter, transform (ttypes: 0 DWT, 1 MODWT) and
+   level count (nlevels, 0 = maximum; MODWT depth <= 8) by universal
+   thresholding of every level; thresh (may be NULL) receives the *count
+   thresholds.  The plan of the last signal table is kept and reused. */
+void RgpuSmoothList(double* x, int* count, int* lens, int* filters, int* ttypes, int* nlevels,
+                    int* hardness, double* thresh);
 
 /* ---- device transforms ------------------------------------------------ */
 
@@ -161,6 +196,22 @@ int wh_coeff_labels(int64_t n, int nlevels, int ttype, int filter, int* level, d
    Synchronous.  No counterpart in the reference, whose GPU MODWT runs one
    kernel per level (src/haarcuda.cu:1649-1914). */
 int wh_modwt_tune(int64_t n, int nlevels, int filter, int dtype, void* stream);
+
+/* Diagnostics of the MODWT planner (no counterpart in the reference).
+   wh_debug_modwt_plan / _plan_many write the level groups a forward
+   (inverse = 0) or inverse launch of 1 / `count` signals runs, as text
+   "{j0= g= R= NT= M= nseg= P= warm= chunks= lds=} ...", into buf.
+   wh_debug_modwt_set_plan sets the planner's choices for one shape as
+   wh_modwt_tune would (deepest first group g1max, segments per CU of the
+   first and deeper groups, 512 or 256 threads in the first inverse group);
+   g1max <= 0 restores the default rules.  Results never depend on the
+   plan; these let a caller run and check every plan the tuner can pick. */
+int wh_debug_modwt_plan(int inverse, int64_t n, int nlevels, int filter, int dtype, char* buf,
+                        int cap);
+int wh_debug_modwt_plan_many(int inverse, int64_t n, int64_t count, int nlevels, int filter,
+                             int dtype, char* buf, int cap);
+int wh_debug_modwt_set_plan(int inverse, int64_t n, int nlevels, int filter, int dtype,
+                            int g1max, int first_per_cu, int deep_per_cu, int first_nt);
 
 /* ---- many signals, adjoints ------------------------------------------ */
 

@@ -143,6 +143,17 @@ def threshold(coeffs, n, nlevels, ttype, filt, t, hardness, minlevel, maxlevel):
     return c
 
 
+def threshold_levels_(c, n, nlevels, ttype, filt, lams, hardness):
+    """In place: the details of 0-based level l thresholded at lams[l], level
+    by level through the oracle's rule (src/thresh.cpp:15-83, :142-157)."""
+    assert c.dtype == np.float64 and c.flags.c_contiguous
+    for l, t in enumerate(lams):
+        rc = oracle().wvo_threshold(_p(c), _p(c), n, nlevels, ttype, filt, float(t), hardness,
+                                    l, l)
+        assert rc == 0, rc
+    return c
+
+
 def median(x):
     x = np.ascontiguousarray(x, dtype=np.float64)
     return oracle().wvo_median(_p(x), len(x))

@@ -332,3 +332,52 @@ def test_cv_rejects_equal_levels():
     t, y, tr, st = O.cvt(x, 5, 1, 2, 1, 0.01, 2, 2)
     assert st != 0
     np.testing.assert_array_equal(y, x)
+
+
+@pytest.mark.parametrize("f", [1, 2, 3, 4])
+def test_modwt_shift_equivariance(f):
+    """Every MODWT position is a DWT coefficient of a shifted signal: with
+    y = x shifted left by r, S_j^x[m 2^j + r] = s_j^y[m] and
+    D_j^x[m 2^j + r] = d_j^y[m].  Together with the golden-pinned DWT this
+    pins the C6 / LA8 MODWT (absent from the reference) at every position,
+    not only at the decimated ones; the term order of the two recursions is
+    the same, so the values agree bit for bit."""
+    n = 256
+    J = O.levels(n, 6, f)
+    x, _ = noisy_doppler(n, 4.0, seed=7)
+    xm = O.modwt_fwd(x, J, f)
+    for r in range(1 << J):
+        y = O.dwt(np.roll(x, -r), J, f)
+        for j in range(1, J + 1):
+            S = xm[2 * n * (j - 1): 2 * n * (j - 1) + n]
+            D = xm[2 * n * (j - 1) + n: 2 * n * j]
+            m = np.arange(n >> j)
+            if (r >> j) == 0:  # r < 2^j: position m 2^j + r of level j
+                np.testing.assert_array_equal(D[(m << j) + r], y[(1 << (j - 1)) + (m << j)])
+                if j == J:
+                    np.testing.assert_array_equal(S[(m << j) + r], y[m << J])
+    # and one shift of the input shifts every block by one
+    xs = O.modwt_fwd(np.roll(x, 1), J, f)
+    for b in range(2 * J):
+        np.testing.assert_array_equal(xs[b * n:(b + 1) * n], np.roll(xm[b * n:(b + 1) * n], 1))
+
+
+@pytest.mark.parametrize("f", [1, 2, 3, 4])
+@pytest.mark.parametrize("n,nl", [(64, 0), (4096, 0), (1 << 16, 10), (1 << 16, 0)])
+def test_openmp_variants_equal_serial(f, n, nl):
+    """The OpenMP restatements (the all-cores CPU baseline) compute every
+    value with the same operations as the serial oracle: bit-identical, DWT
+    and MODWT, forward and inverse."""
+    x, _ = noisy_doppler(n, 5.0, seed=n + f)
+    J = O.levels(n, nl, f)
+    if J == 0:
+        return
+    yd = O.dwt(x, J, f, 1)
+    np.testing.assert_array_equal(O.dwt(x, J, f, 1, omp=True), yd)
+    np.testing.assert_array_equal(O.dwt(yd, J, f, 0, omp=True), O.dwt(yd, J, f, 0))
+    Jm = min(J, 10)
+    xm = O.modwt_fwd(x, Jm, f)
+    np.testing.assert_array_equal(O.modwt_fwd(x, Jm, f, omp=True), xm)
+    rng = np.random.Generator(np.random.PCG64(n))
+    w = rng.standard_normal(xm.size)  # inconsistent coefficients: a left inverse is not pinned by PR
+    np.testing.assert_array_equal(O.modwt_inv(w, n, Jm, f, omp=True), O.modwt_inv(w, n, Jm, f))

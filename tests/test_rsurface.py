@@ -133,3 +133,29 @@ def test_threshold_wst():
     ref = O.threshold(w.w[:2 * n * 5], n, 5, 1, 2, 0.2, 1, 1, 3)
     np.testing.assert_array_equal(wt.w[:2 * n * 5], ref)
     assert (wt.ttype, wt.nlevels, wt.len) == (w.ttype, w.nlevels, w.len)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ttype", ["DWT", "MODWT"])
+def test_cpu_named_functions(ttype):
+    """CPUTransform / CPUReconstruct / CPUThreshold / CPUSmooth over the
+    Rcpu* symbols: the same WST fields and values as the oracle (the
+    reference's CPU path), and the GPU-named functions' results."""
+    n, filt = 2048, "LA8"
+    x, _ = noisy_doppler(n, 3.0, seed=6)
+    nl = 0 if ttype == "DWT" else 5
+    w = R.CPUTransform(x, "FWD", nl, ttype, filt)
+    g = R.GPUTransform(x, "FWD", nl, ttype, filt)
+    J = O.levels(n, nl, 4)
+    assert (w.ttype, w.filt, w.filtlen, w.nlevels, w.len) == (ttype, filt, 8, J, n)
+    np.testing.assert_array_equal(w.w, g.w)
+    want = O.dwt(x, J, 4) if ttype == "DWT" else O.modwt_fwd(x, J, 4)
+    assert O.relerr(w.w[: want.size], want) <= 1e-12
+    assert O.relerr(R.CPUReconstruct(w), x) <= 1e-12
+    wt = R.CPUThreshold(w, 0.2, "hard", 1, J - 1)
+    ref = O.threshold(w.w[: want.size], n, J, 0 if ttype == "DWT" else 1, 4, 0.2, 0, 0, J - 2)
+    np.testing.assert_array_equal(wt.w[: want.size], ref)
+    ys = R.CPUSmooth(x, J, ttype, filt, "univ", None, "soft", 1, J)
+    np.testing.assert_array_equal(ys, R.GPUSmooth(x, J, ttype, filt, "univ", None, "soft", 1, J))
+    yo, _, rc = O.smooth(x, J, 0 if ttype == "DWT" else 1, 4, 1, None, 1, 0, J - 1)
+    assert rc == 0 and O.relerr(ys, yo) <= 1e-12

@@ -64,8 +64,14 @@ SIGNATURES = {
     "wh_interp_mse": (_I, [_VP, _VP, _VP, _I64, _I, _VP, _VP]),
     "wh_smooth": (_I, [_VP, _I64, _I, _I, _I, _I, _I, _DP, _I, _I, _I, C.c_double, _VP]),
     "RgpuTransform": (None, [_DP, _DP, _IP, _IP, _IP, _IP, _IP, _IP]),
+    "RcpuTransform": (None, [_DP, _DP, _IP, _IP, _IP, _IP, _IP, _IP]),
+    "RcpuThreshold": (None, [_DP, _DP, _IP, _IP, _IP, _IP, _IP, _DP, _IP, _IP, _IP]),
     "RgpuSmooth": (None, [_DP, _IP, _IP, _IP, _IP, _IP, _IP, _DP, _IP, _IP, _IP, _DP]),
+    "RcpuSmooth": (None, [_DP, _IP, _IP, _IP, _IP, _IP, _IP, _DP, _IP, _IP, _IP, _DP]),
+    "RgpuSmoothList": (None, [_DP, _IP, _IP, _IP, _IP, _IP, _IP, _DP]),
     "wh_debug_modwt_plan": (_I, [_I, _I64, _I, _I, _I, C.c_char_p, _I]),
+    "wh_debug_modwt_plan_many": (_I, [_I, _I64, _I64, _I, _I, _I, C.c_char_p, _I]),
+    "wh_debug_modwt_set_plan": (_I, [_I, _I64, _I, _I, _I, _I, _I, _I, _I]),
     "wh_modwt_tune": (_I, [_I64, _I, _I, _I, _VP]),
     "wh_coeff_level": (_I, [_VP, _VP, _I64, _I, _I, _I, _I, _I, _I, C.POINTER(_I64), _VP]),
     "wh_modwt_convert": (_I, [_VP, _VP, _I64, _I, _I, _I, _I, _VP]),

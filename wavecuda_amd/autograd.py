@@ -119,6 +119,8 @@ class ThresholdFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, c, lam, n, J, ttype, filt, hardness, minlevel, maxlevel):
         c2, squeeze = _rows(c.contiguous())
+        if c2.shape[1] != (n if ttype == W.DWT else 2 * n * J):
+            raise W.WaveHipError("coefficient rows must hold n (DWT) or 2 n J (MODWT) values")
         lam64 = lam.detach().to(device=c.device, dtype=torch.float64).contiguous()
         if lam64.numel() != J:
             raise W.WaveHipError("one threshold per level expected")

@@ -37,8 +37,19 @@ def _sources():
     return sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
 
 
-def _compile(src: Path) -> Path:
+def _stale(src: Path, obj: Path) -> bool:
+    """The object is missing or older than its source or any header here."""
+    if not obj.exists():
+        return True
+    newest = max([src.stat().st_mtime] + [h.stat().st_mtime for h in CSRC.glob("*.h")] +
+                 [(HERE.parent / "include" / "wavehip.h").stat().st_mtime])
+    return obj.stat().st_mtime < newest
+
+
+def _compile(src: Path, incremental: bool = False) -> Path:
     obj = OBJ / (src.name + ".o")
+    if incremental and not _stale(src, obj):
+        return obj
     cmd = [HIPCC, *CFLAGS, "-x", "hip", "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -46,13 +57,15 @@ def _compile(src: Path) -> Path:
     return obj
 
 
-def build(verbose: bool = False) -> Path:
-    """Compile every source and link the shared library; returns its path."""
+def build(verbose: bool = False, incremental: bool = False) -> Path:
+    """Compile every source (with incremental=True only those whose object
+    is older than the source or a header) and link the shared library;
+    returns its path."""
     OBJ.mkdir(exist_ok=True)
     srcs = _sources()
     jobs = min(len(srcs), max(1, (os.cpu_count() or 4)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda s: _compile(s, incremental), srcs))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -65,5 +78,5 @@ def build(verbose: bool = False) -> Path:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, incremental="--incremental" in sys.argv)
     sys.exit(0)

@@ -29,7 +29,7 @@ struct Scratch {
   void* p = nullptr;
   size_t bytes = 0;
 };
-Scratch g_scratch[kMaxDev][4];
+Scratch g_scratch[kMaxDev][6];
 
 int current_device() {
   int d = -1;
@@ -111,26 +111,28 @@ int scratch_get(size_t bytes, void** out) { return scratch_impl(0, bytes, out); 
 int scratch2_get(size_t bytes, void** out) { return scratch_impl(1, bytes, out); }
 int ws_get(size_t bytes, void** out) { return scratch_impl(2, bytes, out); }
 int stats_get(size_t bytes, void** out) { return scratch_impl(3, bytes, out); }
+int host_buf_get(int which, size_t bytes, void** out) {
+  return scratch_impl(4 + (which & 1), bytes, out);
+}
 
 }  // namespace wh
 
 using namespace wh;
 
-namespace {
+namespace wh {
 
-bool valid_filter(int f) { return f >= kHAAR && f <= kLA8; }
-bool valid_dtype(int t) { return t == WH_F64 || t == WH_F32; }
+bool valid_filter_code(int f) { return f >= kHAAR && f <= kLA8; }
+bool valid_dtype_code(int t) { return t == WH_F64 || t == WH_F32; }
 
 int check_common(const void* a, const void* b, int64_t n, int nlevels, int filter, int dtype,
                  int* J) {
   if (!a || !b) return set_error(WH_ERR_ARG, "null pointer");
-  if (!valid_dtype(dtype)) return set_error(WH_ERR_ARG, "dtype must be WH_F64 or WH_F32");
+  if (!valid_dtype_code(dtype)) return set_error(WH_ERR_ARG, "dtype must be WH_F64 or WH_F32");
   // the kernels move element pairs with single vector accesses
   const uintptr_t pair = dtype == WH_F64 ? 16 : 8;
   if ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & (pair - 1))
     return set_error(WH_ERR_ARG, "device buffers must be aligned to two elements (16 bytes FP64, 8 bytes FP32)");
-  if (!valid_filter(filter)) return set_error(WH_ERR_ARG, "filter must be 1 (Haar), 2 (D4), 3 (C6) or 4 (LA8)");
-  if (!valid_dtype(dtype)) return set_error(WH_ERR_ARG, "dtype must be WH_F64 or WH_F32");
+  if (!valid_filter_code(filter)) return set_error(WH_ERR_ARG, "filter must be 1 (Haar), 2 (D4), 3 (C6) or 4 (LA8)");
   *J = resolve_levels(n, nlevels, filter_length(filter));
   if (*J <= 0) return set_error(WH_ERR_ARG, "length must be a power of two and nlevels <= floor(log2(n/L))+1");
   return WH_OK;
@@ -144,6 +146,16 @@ int check_thresh(const wh_thresh* thr, int J) {
   return WH_OK;
 }
 
+int check_count(int64_t count) {
+  if (count < 1 || count > 65535) return set_error(WH_ERR_ARG, "count must be 1..65535");
+  return WH_OK;
+}
+
+}  // namespace wh
+
+namespace {
+bool valid_filter(int f) { return wh::valid_filter_code(f); }
+bool valid_dtype(int t) { return wh::valid_dtype_code(t); }
 }  // namespace
 
 extern "C" {
@@ -211,11 +223,6 @@ int wh_threshold(void* coeffs, int64_t n, int nlevels, int ttype, int filter, in
   if (!thr || !thr->enabled) return WH_OK;
   if ((rc = check_thresh(thr, J))) return rc;
   return threshold_inplace(coeffs, n, 1, J, ttype, dtype, thr, static_cast<hipStream_t>(stream));
-}
-
-static int check_count(int64_t count) {
-  if (count < 1 || count > 65535) return set_error(WH_ERR_ARG, "count must be 1..65535");
-  return WH_OK;
 }
 
 int wh_modwt_fwd_many(const void* x, void* xmod, int64_t n, int64_t count, int nlevels,
@@ -400,122 +407,31 @@ int wh_modwt_tune(int64_t n, int nlevels, int filter, int dtype, void* stream) {
   return modwt_tune(n, J, filter, dtype, static_cast<hipStream_t>(stream));
 }
 
-// Diagnostic: the level-group plan chosen for a MODWT (written to buf).
-int wh_debug_modwt_plan(int inverse, int64_t n, int nlevels, int filter, int dtype, char* buf,
-                        int cap) {
-  int J = resolve_levels(n, nlevels, filter_length(filter));
-  if (J <= 0 || !buf || cap <= 0) return WH_ERR_ARG;
-  std::string s = modwt_plan_string(inverse != 0, n, J, filter, dtype);
+int wh_debug_modwt_plan_many(int inverse, int64_t n, int64_t count, int nlevels, int filter,
+                             int dtype, char* buf, int cap) {
+  if (!valid_filter(filter) || !valid_dtype(dtype) || !buf || cap <= 0)
+    return set_error(WH_ERR_ARG, "modwt plan: bad filter, dtype or buffer");
+  if (check_count(count)) return WH_ERR_ARG;
+  const int J = resolve_levels(n, nlevels, filter_length(filter));
+  if (J <= 0) return set_error(WH_ERR_ARG, "invalid length / levels");
+  std::string s = modwt_plan_string(inverse != 0, n, J, filter, dtype, count);
   snprintf(buf, size_t(cap), "%s", s.c_str());
   return WH_OK;
 }
 
-// ---------------------------------------------------------------------------
-// reference .C entry points (src/rfunctions.cuh:17-35)
-
-static void report(const char* where, int rc) {
-  fprintf(stderr, "\n Error in GPU %s: %s (code %d)\n", where, wh_last_error(), rc);
+int wh_debug_modwt_plan(int inverse, int64_t n, int nlevels, int filter, int dtype, char* buf,
+                        int cap) {
+  return wh_debug_modwt_plan_many(inverse, n, 1, nlevels, filter, dtype, buf, cap);
 }
 
-struct DevBuf {
-  void* p = nullptr;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-  int alloc(size_t bytes) {
-    if (hipMalloc(&p, bytes) != hipSuccess) {
-      p = nullptr;
-      return set_error(WH_ERR_NOMEM, "device allocation failed");
-    }
-    return WH_OK;
-  }
-};
-
-void RgpuTransform(double* x, double* xmod, int* len, int* sense, int* nlevels, int* ttype,
-                   int* filter, int* filterlen) {
-  (void)filterlen;
-  if (!x || !len || !sense || !nlevels || !ttype || !filter) {
-    report("transform", set_error(WH_ERR_ARG, "null argument"));
-    return;
-  }
-  const int64_t n = *len;
-  const int J = valid_filter(*filter) ? resolve_levels(n, *nlevels, filter_length(*filter)) : 0;
-  if (J <= 0) {
-    report("transform", set_error(WH_ERR_ARG, "invalid length, levels or filter"));
-    return;
-  }
-  int rc = WH_OK;
-  if (*ttype == kDWT) {
-    DevBuf a, b;
-    if ((rc = a.alloc(n * 8)) || (rc = b.alloc(n * 8))) return report("transform", rc);
-    if (hipMemcpy(a.p, x, n * 8, hipMemcpyHostToDevice) != hipSuccess)
-      return report("transform", set_error(WH_ERR_HIP, "copy in"));
-    rc = (*sense == kFWD) ? wh_dwt_fwd(a.p, b.p, n, J, *filter, WH_F64, nullptr)
-                          : wh_dwt_inv(a.p, b.p, n, J, *filter, WH_F64, nullptr, nullptr);
-    if (rc) return report("transform", rc);
-    if (hipMemcpy(x, b.p, n * 8, hipMemcpyDeviceToHost) != hipSuccess)
-      return report("transform", set_error(WH_ERR_HIP, "copy out"));
-    return;
-  }
-  if (*ttype == kMODWT || *ttype == kMODWT_PO) {
-    // packet order (ttype 2, the reference's CPU dispatch of
-    // src/transform.cpp:39-55) is the time-ordered transform plus the
-    // conversion of src/wavutils.cpp:532-600 -- bit-identical to the
-    // reference's own packet-ordered forward
-    if (!xmod) return report("transform", set_error(WH_ERR_ARG, "null xmod"));
-    const bool po = *ttype == kMODWT_PO;
-    const size_t nm = size_t(2) * size_t(n) * size_t(J);
-    DevBuf a, b, c;
-    if ((rc = a.alloc(n * 8)) || (rc = b.alloc(nm * 8)) || (po && (rc = c.alloc(nm * 8))))
-      return report("transform", rc);
-    if (*sense == kFWD) {
-      if (hipMemcpy(a.p, x, n * 8, hipMemcpyHostToDevice) != hipSuccess)
-        return report("transform", set_error(WH_ERR_HIP, "copy in"));
-      if ((rc = wh_modwt_fwd(a.p, b.p, n, J, *filter, WH_F64, nullptr))) return report("transform", rc);
-      if (po && (rc = wh_modwt_convert(b.p, c.p, n, J, *filter, 1, WH_F64, nullptr)))
-        return report("transform", rc);
-      if (hipMemcpy(xmod, po ? c.p : b.p, nm * 8, hipMemcpyDeviceToHost) != hipSuccess)
-        return report("transform", set_error(WH_ERR_HIP, "copy out"));
-    } else {
-      if (hipMemcpy(po ? c.p : b.p, xmod, nm * 8, hipMemcpyHostToDevice) != hipSuccess)
-        return report("transform", set_error(WH_ERR_HIP, "copy in"));
-      if (po && (rc = wh_modwt_convert(c.p, b.p, n, J, *filter, 0, WH_F64, nullptr)))
-        return report("transform", rc);
-      if ((rc = wh_modwt_inv(b.p, a.p, n, J, *filter, WH_F64, nullptr, nullptr)))
-        return report("transform", rc);
-      if (hipMemcpy(x, a.p, n * 8, hipMemcpyDeviceToHost) != hipSuccess)
-        return report("transform", set_error(WH_ERR_HIP, "copy out"));
-    }
-    return;
-  }
-  report("transform", set_error(WH_ERR_ARG, "ttype must be 0 (DWT), 1 (MODWT) or 2 (packet-ordered MODWT)"));
-}
-
-void RgpuSmooth(double* x, int* len, int* nlevels, int* ttype, int* filter, int* filterlen,
-                int* threshtype, double* thresh, int* hardness, int* minlevel, int* maxlevel,
-                double* tol) {
-  (void)filterlen;
-  if (!x || !len || !nlevels || !ttype || !filter || !threshtype || !hardness || !minlevel ||
-      !maxlevel) {
-    report("smooth", set_error(WH_ERR_ARG, "null argument"));
-    return;
-  }
-  const int64_t n = *len;
-  const int J = valid_filter(*filter) ? resolve_levels(n, *nlevels, filter_length(*filter)) : 0;
-  if (J <= 0) return report("smooth", set_error(WH_ERR_ARG, "invalid length, levels or filter"));
-  DevBuf a;
-  int rc;
-  if ((rc = a.alloc(n * 8))) return report("smooth", rc);
-  if (hipMemcpy(a.p, x, n * 8, hipMemcpyHostToDevice) != hipSuccess)
-    return report("smooth", set_error(WH_ERR_HIP, "copy in"));
-  double t = thresh ? *thresh : 0.0;
-  const double tl = tol ? *tol : 0.01;
-  if ((rc = wh_smooth(a.p, n, J, *ttype, *filter, WH_F64, *threshtype, &t, *hardness, *minlevel,
-                      *maxlevel, tl, nullptr)))
-    return report("smooth", rc);
-  if (hipMemcpy(x, a.p, n * 8, hipMemcpyDeviceToHost) != hipSuccess)
-    return report("smooth", set_error(WH_ERR_HIP, "copy out"));
-  if (thresh && *threshtype != 0) *thresh = t;
+int wh_debug_modwt_set_plan(int inverse, int64_t n, int nlevels, int filter, int dtype,
+                            int g1max, int first_per_cu, int deep_per_cu, int first_nt) {
+  if (!valid_filter(filter)) return set_error(WH_ERR_ARG, "filter must be 1..4");
+  if (!valid_dtype(dtype)) return set_error(WH_ERR_ARG, "dtype must be WH_F64 or WH_F32");
+  const int J = resolve_levels(n, nlevels, filter_length(filter));
+  if (J <= 0) return set_error(WH_ERR_ARG, "invalid length / levels");
+  return modwt_force_plan(inverse != 0, n, J, filter, dtype, g1max, first_per_cu, deep_per_cu,
+                          first_nt);
 }
 
 }  // extern "C"

@@ -238,18 +238,25 @@ int modwt_fwd_t(const void* x, void* xmod, int64_t n, int64_t count, int J, int 
 
 template <typename T>
 int modwt_inv_t(const void* xmod, void* x, int64_t n, int64_t count, int J, int filt,
-                const wh_thresh* thr, hipStream_t st) {
+                const wh_thresh* thr, hipStream_t st, void* ws = nullptr, size_t ws_bytes = 0) {
   const int L = filter_length(filt);
   auto groups = plan_groups(true, n, J, L, sizeof(T), knobs_for(true, n, J, filt, sizeof(T), count),
                             count);
   const int64_t per = 2 * n * J;
-  // the S_{j0} handed between groups lives in library scratch (2 n values
-  // per signal: two halves used alternately)
+  // the S_{j0} handed between groups lives in library scratch, or in the
+  // caller's workspace `ws` (2 n values per signal: two halves used
+  // alternately; every group writes all of its half before the next group
+  // reads it, so nothing needs clearing)
   T* scratch = nullptr;
   if (groups.size() > 1) {
-    void* p = nullptr;
-    int rc = scratch_get(2 * size_t(n) * size_t(count) * sizeof(T), &p);
-    if (rc) return rc;
+    const size_t need = 2 * size_t(n) * size_t(count) * sizeof(T);
+    void* p = ws;
+    if (ws) {
+      if (need > ws_bytes) return set_error(WH_ERR_ARG, "modwt inverse: workspace too small");
+    } else {
+      int rc = scratch_get(need, &p);
+      if (rc) return rc;
+    }
     scratch = static_cast<T*>(p);
   }
   const T* stop = static_cast<const T*>(xmod) + 2 * n * int64_t(J - 1);
@@ -292,6 +299,13 @@ int modwt_inverse(const void* xmod, void* x, int64_t n, int64_t count, int J, in
                   const wh_thresh* thr, hipStream_t st) {
   return dtype == WH_F32 ? modwt_inv_t<float>(xmod, x, n, count, J, filt, thr, st)
                          : modwt_inv_t<double>(xmod, x, n, count, J, filt, thr, st);
+}
+
+int modwt_inverse_ws(const void* xmod, void* x, int64_t n, int64_t count, int J, int filt,
+                     int dtype, const wh_thresh* thr, void* ws, size_t ws_bytes, hipStream_t st) {
+  if (!ws) return set_error(WH_ERR_ARG, "modwt inverse: null workspace");
+  return dtype == WH_F32 ? modwt_inv_t<float>(xmod, x, n, count, J, filt, thr, st, ws, ws_bytes)
+                         : modwt_inv_t<double>(xmod, x, n, count, J, filt, thr, st, ws, ws_bytes);
 }
 
 int modwt_inverse_adjoint(const void* gx, void* gxmod, int64_t n, int64_t count, int J, int filt,
@@ -433,10 +447,31 @@ int modwt_tune(int64_t n, int J, int filt, int dtype, hipStream_t st) {
   return dtype == WH_F32 ? tune_t<float>(n, J, filt, st) : tune_t<double>(n, J, filt, st);
 }
 
-std::string modwt_plan_string(bool inverse, int64_t n, int J, int filt, int dtype) {
+int modwt_force_plan(bool inverse, int64_t n, int J, int filt, int dtype, int g1max,
+                     int first_per_cu, int deep_per_cu, int first_nt) {
+  const KnobKey key{inverse ? 1 : 0, n, J, filt, dtype == WH_F32 ? 4 : 8};
+  std::lock_guard<std::mutex> lk(g_knob_mu);
+  if (g1max <= 0) {
+    g_knobs.erase(key);
+    return WH_OK;
+  }
+  if (g1max > kGmax || first_per_cu < 1 || deep_per_cu < 1 ||
+      (first_nt != kNT1 && first_nt != kNT1 / 2) || (!inverse && first_nt != kNT1))
+    return set_error(WH_ERR_ARG, "modwt plan: knobs out of range");
+  PlanKnobs k;
+  k.g1max = g1max;
+  k.first_per_cu = first_per_cu;
+  k.deep_per_cu = deep_per_cu;
+  k.first_nt = first_nt;
+  g_knobs[key] = k;
+  return WH_OK;
+}
+
+std::string modwt_plan_string(bool inverse, int64_t n, int J, int filt, int dtype,
+                              int64_t count) {
   const int L = filter_length(filt);
   size_t esz = dtype == WH_F32 ? 4 : 8;
-  auto gs = plan_groups(inverse, n, J, L, esz, knobs_for(inverse, n, J, filt, esz));
+  auto gs = plan_groups(inverse, n, J, L, esz, knobs_for(inverse, n, J, filt, esz, count), count);
   std::string s;
   for (auto& G : gs) {
     char buf[256];

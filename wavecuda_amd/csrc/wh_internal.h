@@ -32,6 +32,20 @@ int scratch_get(size_t bytes, void** out);
 int scratch2_get(size_t bytes, void** out);
 int ws_get(size_t bytes, void** out);
 int stats_get(size_t bytes, void** out);
+// Two more regions (which = 0, 1) for the device copies of the host-pointer
+// entry points' arguments, kept across calls like the others.
+int host_buf_get(int which, size_t bytes, void** out);
+
+// Argument checks shared by the entry points (status WH_ERR_ARG with a
+// message on failure): codes, two-element alignment of a and b, the
+// resolved level count *J; a threshold specification against J levels; a
+// signal count of 1..65535.
+bool valid_filter_code(int f);
+bool valid_dtype_code(int t);
+int check_common(const void* a, const void* b, int64_t n, int nlevels, int filter, int dtype,
+                 int* J);
+int check_thresh(const wh_thresh* thr, int J);
+int check_count(int64_t count);
 
 // Let kernel `kern` take all of the current device's LDS beyond its static
 // LDS as dynamic LDS; done once per (device, kernel), so it is cheap to call
@@ -45,6 +59,10 @@ int modwt_forward(const void* x, void* xmod, int64_t n, int64_t count, int J, in
                   int dtype, hipStream_t st);
 int modwt_inverse(const void* xmod, void* x, int64_t n, int64_t count, int J, int filt, int dtype,
                   const wh_thresh* thr, hipStream_t st);
+// the same with the level hand-off in the caller's device workspace ws of
+// ws_bytes (2 n count elements needed when the plan has several groups)
+int modwt_inverse_ws(const void* xmod, void* x, int64_t n, int64_t count, int J, int filt,
+                     int dtype, const wh_thresh* thr, void* ws, size_t ws_bytes, hipStream_t st);
 // adjoints: of the inverse (x-cotangent -> xmod-cotangent, the fused forward
 // with exact 2^-j scalings) and of the forward (xmod-cotangent -> x-cotangent)
 int modwt_inverse_adjoint(const void* gx, void* gxmod, int64_t n, int64_t count, int J, int filt,
@@ -53,10 +71,16 @@ int modwt_forward_adjoint(const void* gxmod, void* gx, int64_t n, int64_t count,
                           int dtype, hipStream_t st);
 int thresh_grad(const void* c, const void* gy, void* gc, double* glam, int64_t n, int64_t count,
                 int J, int ttype, int dtype, const wh_thresh* thr, hipStream_t st);
-std::string modwt_plan_string(bool inverse, int64_t n, int J, int filt, int dtype);
+// the level groups a launch of `count` signals runs, as text
+std::string modwt_plan_string(bool inverse, int64_t n, int J, int filt, int dtype,
+                              int64_t count = 1);
 // time the planner's alternatives for (n, J, filter, element type) and keep
 // the fastest forward and inverse plans for the rest of the process
 int modwt_tune(int64_t n, int J, int filt, int dtype, hipStream_t st);
+// set the planner's knobs for (n, J, filter, element type) as the tuner
+// would (g1max <= 0: back to the default rules)
+int modwt_force_plan(bool inverse, int64_t n, int J, int filt, int dtype, int g1max,
+                     int first_per_cu, int deep_per_cu, int first_nt);
 
 // levels.hip: one level's coefficients gathered into out (count written to
 // *count_out), and the per-coefficient label columns

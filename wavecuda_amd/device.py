@@ -37,6 +37,21 @@ def _check_dev(*ts):
             raise ValueError("tensors must be contiguous")
 
 
+def _need(t: torch.Tensor, count: int, dtype, what: str):
+    """t must be a contiguous GPU tensor of `dtype` holding at least `count`
+    values: the kernels read or write that many through its raw pointer."""
+    _check_dev(t)
+    if t.dtype != dtype:
+        raise ValueError(f"{what}: dtype {t.dtype} does not match {dtype}")
+    if t.numel() < count:
+        raise ValueError(f"{what}: holds {t.numel()} values, needs {count}")
+
+
+def _coeff_count(ttype: int, n: int, J: int) -> int:
+    """Values of a transformed vector: n (DWT) or 2 n J (MODWT)."""
+    return n if int(ttype) == 0 else 2 * n * J
+
+
 def _stream():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
@@ -48,7 +63,7 @@ def resolve_levels(n: int, nlevels: int, filter) -> int:
     return J
 
 
-def _thresh_struct(thresh):
+def _thresh_struct(thresh, J=None):
     if thresh is None:
         return None, None
     if isinstance(thresh, W.wh_thresh):
@@ -65,6 +80,8 @@ def _thresh_struct(thresh):
     if levels is not None:
         keep = levels.to(dtype=torch.float64).contiguous()
         _check_dev(keep)
+        if J is not None and keep.numel() < J:
+            raise ValueError(f"per-level thresholds: {keep.numel()} values for {J} levels")
         ptr = keep.data_ptr()
     t = W.make_thresh(True, hardness, minlevel, maxlevel, value, ptr)
     return t, keep
@@ -78,7 +95,7 @@ def modwt_fwd(x: torch.Tensor, nlevels: int = 0, filter="LA8", out: torch.Tensor
     J = resolve_levels(n, nlevels, f)
     if out is None:
         out = torch.empty(2 * n * J, dtype=x.dtype, device=x.device)
-    _check_dev(out)
+    _need(out, 2 * n * J, x.dtype, "modwt_fwd out")
     W.check(W.lib().wh_modwt_fwd(x.data_ptr(), out.data_ptr(), n, J, f, _dtype(x), _stream()),
             "wh_modwt_fwd")
     return out
@@ -97,8 +114,8 @@ def modwt_inv(xmod: torch.Tensor, n: int, nlevels: int = 0, filter="LA8", thresh
         raise ValueError("xmod is shorter than 2*n*nlevels")
     if out is None:
         out = torch.empty(n, dtype=xmod.dtype, device=xmod.device)
-    _check_dev(out)
-    t, keep = _thresh_struct(thresh)
+    _need(out, n, xmod.dtype, "modwt_inv out")
+    t, keep = _thresh_struct(thresh, J)
     W.check(
         W.lib().wh_modwt_inv(xmod.data_ptr(), out.data_ptr(), n, J, f, _dtype(xmod),
                              C.byref(t) if t is not None else None, _stream()),
@@ -115,7 +132,7 @@ def dwt_fwd(x: torch.Tensor, nlevels: int = 0, filter="LA8", out: torch.Tensor |
     J = resolve_levels(n, nlevels, f)
     if out is None:
         out = torch.empty_like(x)
-    _check_dev(out)
+    _need(out, n, x.dtype, "dwt_fwd out")
     W.check(W.lib().wh_dwt_fwd(x.data_ptr(), out.data_ptr(), n, J, f, _dtype(x), _stream()),
             "wh_dwt_fwd")
     return out
@@ -130,8 +147,8 @@ def dwt_inv(y: torch.Tensor, nlevels: int = 0, filter="LA8", thresh=None,
     J = resolve_levels(n, nlevels, f)
     if out is None:
         out = torch.empty_like(y)
-    _check_dev(out)
-    t, keep = _thresh_struct(thresh)
+    _need(out, n, y.dtype, "dwt_inv out")
+    t, keep = _thresh_struct(thresh, J)
     W.check(
         W.lib().wh_dwt_inv(y.data_ptr(), out.data_ptr(), n, J, f, _dtype(y),
                            C.byref(t) if t is not None else None, _stream()),
@@ -149,12 +166,49 @@ def modwt_tune(n: int, nlevels: int, filter="LA8", dtype=torch.float64) -> None:
                                   _stream()), "wh_modwt_tune")
 
 
-def modwt_plan(inverse: bool, n: int, nlevels: int, filter="LA8", dtype=torch.float64) -> str:
+def modwt_plan(inverse: bool, n: int, nlevels: int, filter="LA8", dtype=torch.float64,
+               count: int = 1) -> str:
+    """The level groups the fused MODWT runs for this shape (a launch of
+    `count` signals), as text."""
     buf = C.create_string_buffer(4096)
-    rc = W.lib().wh_debug_modwt_plan(1 if inverse else 0, n, nlevels, filter_code(filter),
-                                    _DT[dtype], buf, 4096)
-    W.check(rc, "wh_debug_modwt_plan")
+    rc = W.lib().wh_debug_modwt_plan_many(1 if inverse else 0, n, count, nlevels,
+                                         filter_code(filter), _DT[dtype], buf, 4096)
+    W.check(rc, "wh_debug_modwt_plan_many")
     return buf.value.decode()
+
+
+def modwt_set_plan(inverse: bool, n: int, nlevels: int, filter="LA8", dtype=torch.float64,
+                   knobs=None) -> None:
+    """Set the planner's choices for one shape as the tuner would: knobs =
+    (deepest first group, segments per CU of the first group, of deeper
+    groups, threads of the first inverse group); None restores the default
+    rules."""
+    g1, f, d, nt = knobs if knobs is not None else (0, 1, 1, 512)
+    W.check(W.lib().wh_debug_modwt_set_plan(1 if inverse else 0, n, nlevels, filter_code(filter),
+                                            _DT[dtype], g1, f, d, nt), "wh_debug_modwt_set_plan")
+
+
+def modwt_plan_candidates(inverse: bool, n: int, nlevels: int, filter="LA8",
+                          dtype=torch.float64):
+    """Every distinct plan the tuner times for this shape: {plan text: knobs}
+    (the tuner's search space: deepest first group min(J, 5)..10, segments
+    per CU of the first group 1/2/4 and of deeper groups 1/2/4/8, and for
+    the inverse a 256-thread first group).  Leaves the default rules in
+    force."""
+    f = filter_code(filter)
+    J = resolve_levels(n, nlevels, f)
+    out = {}
+    try:
+        for nt in ((512, 256) if inverse else (512,)):
+            for g1 in range(min(J, 5), 11):
+                for fp in (1, 2, 4):
+                    for dp in (1, 2, 4, 8):
+                        k = (g1, fp, dp, nt)
+                        modwt_set_plan(inverse, n, J, f, dtype, k)
+                        out.setdefault(modwt_plan(inverse, n, J, f, dtype), k)
+    finally:
+        modwt_set_plan(inverse, n, J, f, dtype, None)
+    return out
 
 
 def modwt_convert(xmod: torch.Tensor, n: int, nlevels: int, filter="LA8", to_po: bool = True,
@@ -168,7 +222,7 @@ def modwt_convert(xmod: torch.Tensor, n: int, nlevels: int, filter="LA8", to_po:
         raise ValueError(f"xmod must hold 2 n J = {2 * n * J} values")
     if out is None:
         out = torch.empty(2 * n * J, dtype=xmod.dtype, device=xmod.device)
-    _check_dev(out)
+    _need(out, 2 * n * J, xmod.dtype, "modwt_convert out")
     W.check(W.lib().wh_modwt_convert(xmod.data_ptr(), out.data_ptr(), n, J, f,
                                      1 if to_po else 0, _dtype(xmod), _stream()),
             "wh_modwt_convert")
@@ -198,6 +252,7 @@ def coeff_level(coeffs: torch.Tensor, n: int, nlevels: int, ttype: int, filter, 
     J = resolve_levels(n, nlevels, f)
     if coeff_type not in ("d", "s"):
         raise ValueError('coeff_type must be "d" or "s"')
+    _need(coeffs, _coeff_count(ttype, n, J), coeffs.dtype, "coeff_level coeffs")
     if not 1 <= level <= J:
         raise ValueError(f"level must be between 1 and {J}")
     s = coeff_type == "s"
@@ -232,8 +287,9 @@ def threshold_(coeffs: torch.Tensor, n: int, nlevels: int, ttype: int, filter, v
     _check_dev(coeffs)
     f = filter_code(filter)
     J = resolve_levels(n, nlevels, f)
+    _need(coeffs, _coeff_count(ttype, n, J), coeffs.dtype, "threshold_ coeffs")
     t, keep = _thresh_struct(dict(value=value, hardness=hardness, minlevel=minlevel,
-                                  maxlevel=maxlevel, levels=levels))
+                                  maxlevel=maxlevel, levels=levels), J)
     W.check(W.lib().wh_threshold(coeffs.data_ptr(), n, J, int(ttype), f, _dtype(coeffs),
                                  C.byref(t), _stream()), "wh_threshold")
     del keep
@@ -263,12 +319,18 @@ def univ_thresh(coeffs: torch.Tensor, n: int, nlevels: int, ttype: int, filter,
     _check_dev(coeffs)
     f = filter_code(filter)
     J = resolve_levels(n, nlevels, f)
+    _need(coeffs, _coeff_count(ttype, n, J), coeffs.dtype, "univ_thresh coeffs")
     return _scalar_out(W.lib().wh_univ_thresh, "wh_univ_thresh", coeffs.data_ptr(), n, J,
                        int(ttype), f, _dtype(coeffs), int(minlevel), int(maxlevel))
 
 
 def interp_mse(xn: torch.Tensor, ye: torch.Tensor, yo: torch.Tensor) -> torch.Tensor:
     _check_dev(xn, ye, yo)
+    h = xn.numel() // 2
+    for t, name in ((ye, "ye"), (yo, "yo")):
+        _need(t, h, xn.dtype, f"interp_mse {name}")
+        if t.numel() != h:
+            raise ValueError(f"interp_mse {name}: holds {t.numel()} values, needs n/2 = {h}")
     return _scalar_out(W.lib().wh_interp_mse, "wh_interp_mse", xn.data_ptr(), ye.data_ptr(),
                        yo.data_ptr(), xn.numel(), _dtype(xn))
 

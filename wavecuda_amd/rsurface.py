@@ -1,7 +1,8 @@
 """The reference package's R surface, restated in Python over the `.C` symbols.
 
 GPUTransform, GPUReconstruct, GPUSmooth (R/rfunctions.R:383-406, :452-468,
-:588-612) and a GPU counterpart of CPUThreshold (:508-526), with the same
+:588-612), CPUTransform, CPUReconstruct, CPUThreshold, CPUSmooth (:334-357,
+:421-437, :508-526, :547-567) and a GPU counterpart of CPUThreshold, with the same
 argument names (R's dots become underscores), the same checks and error
 messages (check.trans.inputs :64-119, check.thresh.inputs :168-202,
 check.smooth.inputs :211-249) and the same returned WST object (return.trans
@@ -9,8 +10,10 @@ check.smooth.inputs :211-249) and the same returned WST object (return.trans
 len, w, where a MODWT's w is the 2 n log2(n) vector R allocates
 (:94) with the 2 n J coefficients first.
 
-The transforms and smoothing run through `RgpuTransform` / `RgpuSmooth`
-called exactly the way R's `.C` calls them (host double vectors, pointers
+The transforms and smoothing run through the `.C` symbols each R function
+binds (RgpuTransform / RgpuSmooth for the GPU functions, RcpuTransform /
+RcpuThreshold / RcpuSmooth for the CPU-named ones, which this library also
+runs on the GPU) called exactly the way R's `.C` calls them (host double vectors, pointers
 to one int each, results written in place), so this module also exercises
 the drop-in boundary.  Differences from the reference, all additions: C6
 and LA8 MODWT are accepted (the reference stops at :114-116), and
@@ -168,12 +171,12 @@ def _ip(v):
     return C.byref(C.c_int(int(v)))
 
 
-def _call_transform(a):
-    """.C("RgpuTransform", x, xmod, len, sense, nlevels, ttype, filter,
-    filterlen) -- R/rfunctions.R:394-403."""
+def _call_transform(a, sym="RgpuTransform"):
+    """.C(sym, x, xmod, len, sense, nlevels, ttype, filter, filterlen) --
+    R/rfunctions.R:345-354 (RcpuTransform), :394-403 (RgpuTransform)."""
     x = np.ascontiguousarray(a["x"], dtype=np.float64)
     xmod = np.ascontiguousarray(a["xmod"], dtype=np.float64)
-    W.lib().RgpuTransform(_dp(x), _dp(xmod) if xmod.size else None, _ip(a["len"]),
+    getattr(W.lib(), sym)(_dp(x), _dp(xmod) if xmod.size else None, _ip(a["len"]),
                           _ip(a["sense"]), _ip(a["nlevels"]), _ip(a["ttype"]), _ip(a["filt"]),
                           _ip(a["filtlen"]))
     out = dict(a)
@@ -205,6 +208,39 @@ def GPUReconstruct(xwav: WST):
     return _call_transform(prepare_wst_arg_list(xwav))["x"]
 
 
+def CPUTransform(xin, direction="FWD", nlevels=0, transform_type="DWT", filter="Haar"):
+    """CPUTransform (R/rfunctions.R:334-357) over RcpuTransform."""
+    args_in = {"ttype": transform_type, "filter": filter}
+    if direction == "FWD":
+        args_in["x"] = np.asarray(xin)
+    a = check_trans_inputs(xin, direction, nlevels, transform_type, filter)
+    return return_trans(_call_transform(a, "RcpuTransform"), args_in)
+
+
+def CPUReconstruct(xwav: WST):
+    """CPUReconstruct (R/rfunctions.R:421-437) over RcpuTransform."""
+    return _call_transform(prepare_wst_arg_list(xwav), "RcpuTransform")["x"]
+
+
+def return_thresh(xwav: WST, arg_list):
+    """return.thresh (R/rfunctions.R:300-308)."""
+    out = WST(**{**xwav.__dict__})
+    out.w = arg_list["x"] if xwav.ttype == "DWT" else arg_list["xmod"]
+    return out
+
+
+def CPUThreshold(xwav: WST, thresh, hard_soft, min_level, max_level):
+    """CPUThreshold (R/rfunctions.R:508-526) over RcpuThreshold."""
+    a = check_thresh_inputs(xwav, hard_soft, thresh, min_level, max_level)
+    x = np.ascontiguousarray(a["x"], dtype=np.float64)
+    xmod = np.ascontiguousarray(a["xmod"], dtype=np.float64)
+    t = np.array([a["thresh"]])
+    W.lib().RcpuThreshold(_dp(x), _dp(xmod) if xmod.size else None, _ip(xwav.len),
+                          _ip(xwav.nlevels), _ip(a["ttype"]), _ip(a["filt"]), _ip(a["filtlen"]),
+                          _dp(t), _ip(a["hardness"]), _ip(a["min_level"]), _ip(a["max_level"]))
+    return return_thresh(xwav, {"x": x, "xmod": xmod})
+
+
 def GPUThreshold(xwav: WST, thresh, hard_soft, min_level, max_level):
     """The GPU counterpart of CPUThreshold (R/rfunctions.R:508-526): the
     WST with its coefficients thresholded on the device."""
@@ -221,8 +257,15 @@ def GPUThreshold(xwav: WST, thresh, hard_soft, min_level, max_level):
     return out
 
 
-def GPUSmooth(xin, nlevels, transform_type, filter, thresh_type, thresh=None, hard_soft="soft",
+def CPUSmooth(xin, nlevels, transform_type, filter, thresh_type, thresh=None, hard_soft="soft",
               min_level=1, max_level=None, tol=0.01):
+    """CPUSmooth (R/rfunctions.R:547-567) over RcpuSmooth: the smoothed vector."""
+    return GPUSmooth(xin, nlevels, transform_type, filter, thresh_type, thresh, hard_soft,
+                     min_level, max_level, tol, _sym="RcpuSmooth")
+
+
+def GPUSmooth(xin, nlevels, transform_type, filter, thresh_type, thresh=None, hard_soft="soft",
+              min_level=1, max_level=None, tol=0.01, _sym="RgpuSmooth"):
     """GPUSmooth (R/rfunctions.R:588-612): the smoothed vector."""
     if max_level is None:
         max_level = nlevels
@@ -231,9 +274,9 @@ def GPUSmooth(xin, nlevels, transform_type, filter, thresh_type, thresh=None, ha
     x = np.ascontiguousarray(a["x"], dtype=np.float64)
     t = np.array([a["thresh"] if a["thresh"] is not None else 0.0])
     tl = np.array([a["tol"] if a["tol"] is not None else 0.01])
-    W.lib().RgpuSmooth(_dp(x), _ip(a["len"]), _ip(a["nlevels"]), _ip(a["ttype"]), _ip(a["filt"]),
-                       _ip(a["filtlen"]), _ip(a["threshtype"]), _dp(t), _ip(a["hardness"]),
-                       _ip(a["min_level"]), _ip(a["max_level"]), _dp(tl))
+    getattr(W.lib(), _sym)(_dp(x), _ip(a["len"]), _ip(a["nlevels"]), _ip(a["ttype"]),
+                           _ip(a["filt"]), _ip(a["filtlen"]), _ip(a["threshtype"]), _dp(t),
+                           _ip(a["hardness"]), _ip(a["min_level"]), _ip(a["max_level"]), _dp(tl))
     return x
 
 
