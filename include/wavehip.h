@@ -74,7 +74,8 @@ typedef struct wh_thresh {
 const char* wh_last_error(void);
 /* Library version, major*10000 + minor*100 + patch. */
 int wh_version(void);
-/* Optional: select and initialise a device (the current one when < 0). */
+/* Optional: select and initialise a device (the current one when < 0);
+   also places wh_modwt_inv_noalloc's static scratch on it. */
 int wh_init(int device);
 /* Resolved level count: reference src/wavutils.cpp:48-65, 0 on error. */
 int wh_resolve_levels(int64_t n, int nlevels, int filter);
@@ -223,6 +224,19 @@ int wh_modwt_fwd_many(const void* x, void* xmod, int64_t n, int64_t count, int n
                       int filter, int dtype, void* stream);
 int wh_modwt_inv_many(const void* xmod, void* x, int64_t n, int64_t count, int nlevels,
                       int filter, int dtype, const wh_thresh* thr, void* stream);
+/* wh_modwt_inv_many without any device allocation.  No counterpart in the
+   reference; it is for callers that cannot allocate (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc).  The
+   inverse's level groups hand S_j between them through a fixed 16 MiB
+   device array that each GPU holds from wh_init (or this function's first
+   call there) until the process ends, instead of the grown library
+   scratch: n * count may be at most 2^20 (FP64) or 2^21 (FP32), larger
+   inputs return WH_ERR_ARG.  Every call on a GPU shares that GPU's array
+   (it needs no clearing: each group writes its part before the next reads
+   it), so calls that overlap in time on one GPU race: order them on one
+   stream.  Same results as wh_modwt_inv_many, bit for bit. */
+int wh_modwt_inv_noalloc(const void* xmod, void* x, int64_t n, int64_t count, int nlevels,
+                         int filter, int dtype, const wh_thresh* thr, void* stream);
 /* Adjoints (vector-Jacobian products) for differentiation; no counterpart
    in the reference, derived from the transforms above.
    wh_modwt_fwd_adj: cotangent of xmod (every S_j and D_j) -> cotangent of x.

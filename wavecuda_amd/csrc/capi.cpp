@@ -177,7 +177,7 @@ int wh_init(int device) {
     std::string m = "device arch " + di.arch + " is not gfx950";
     return set_error(WH_ERR_NODEVICE, m.c_str());
   }
-  return WH_OK;
+  return noalloc_setup();
 }
 
 int wh_resolve_levels(int64_t n, int nlevels, int filter) {

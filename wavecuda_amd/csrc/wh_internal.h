@@ -47,6 +47,10 @@ int check_common(const void* a, const void* b, int64_t n, int nlevels, int filte
 int check_thresh(const wh_thresh* thr, int J);
 int check_count(int64_t count);
 
+// The static scratch of wh_modwt_inv_noalloc placed on the current device
+// and its address looked up (wh_init calls it).
+int noalloc_setup();
+
 // Let kernel `kern` take all of the current device's LDS beyond its static
 // LDS as dynamic LDS; done once per (device, kernel), so it is cheap to call
 // before every launch.  A refused request is cleared from HIP's last error
