@@ -113,6 +113,80 @@ def cpu_baseline(n_sample=1 << 22, J=10, filt=4, omp=False, budget_s=10.0):
     }
 
 
+def cpu_config1(budget_s=1.0):
+    """The oracle's D4 DWT forward + inverse of the n = 1024 doppler signal
+    (9 levels, full depth), single thread, timed in C (one transform is a
+    few microseconds)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import numpy as np
+
+    import oracle_bind as O  # test infrastructure: the checker, timed as a baseline
+    from wavecuda_amd.signals import noisy_doppler
+
+    x = noisy_doppler(1024, 5.0, seed=10)[0]
+    w = np.zeros(1024)
+    reps = 1000
+    t = O.oracle().wvo_time_dwt_fwd_inv(O._p(x), O._p(w), 1024, 9, 2, reps)
+    while t < budget_s:
+        reps *= 2
+        t = O.oracle().wvo_time_dwt_fwd_inv(O._p(x), O._p(w), 1024, 9, 2, reps)
+    return {"value": round(t / reps * 1e6, 3), "unit": "us per fwd+inv", "cores": 1,
+            "kind": "port",
+            "sample": f"D4 DWT fwd+inv, n=1024, 9 levels, doppler+noise SNR 5, {reps} reps, "
+                      "single thread, oracle/wvoracle.c"}
+
+
+def cpu_config3():
+    """The oracle's CPUSmooth on every smoothing case the GPU section runs
+    (n = 2^18 doppler + noise, MODWT J = 10, Haar / LA8, universal / CV,
+    soft / hard, SNR 5 / 2), single thread, one call each."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_bind as O
+    from wavecuda_amd.signals import noisy_doppler
+
+    n = 1 << 18
+    rows = []
+    for snr in (5.0, 2.0):
+        xs, _ = noisy_doppler(n, snr, seed=10)
+        for f, fname in ((1, "Haar"), (4, "LA8")):
+            for tt, tname in ((1, "univ"), (2, "cv")):
+                for hard in (1, 0):
+                    t0 = time.perf_counter()
+                    _, th, rc = O.smooth(xs, 10, 1, f, tt, None, hard, 0, 9)
+                    dt = time.perf_counter() - t0
+                    rows.append({"snr": snr, "filter": fname, "thresh": tname,
+                                 "hardness": "soft" if hard else "hard",
+                                 "ms": round(dt * 1e3, 2), "threshold": th, "rc": rc})
+    return {"unit": "ms per smoothing call", "cores": 1, "kind": "port",
+            "sample": "all 16 cases of the smoothing section, one call each, single thread, "
+                      "oracle/wvoracle.c (the restated CPUSmooth path)",
+            "cases": rows}
+
+
+def cpu_config4(sample=1024):
+    """The oracle's universal-threshold smoothing, signal by signal, over the
+    first `sample` signals of the 8192-signal mixed batch, single thread."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import numpy as np
+
+    import oracle_bind as O
+    from wavecuda_amd.signals import batch_workload
+
+    x, lens, filters, ttypes, levels = batch_workload()
+    off = np.concatenate([[0], np.cumsum(lens.astype(np.int64))])
+    m = min(sample, len(lens))
+    t0 = time.perf_counter()
+    for i in range(m):
+        J = O.levels(int(lens[i]), int(levels[i]), int(filters[i]))
+        O.smooth(x[off[i]:off[i + 1]], J, int(ttypes[i]), int(filters[i]), 1, None, 1, 0, J - 1)
+    dt = time.perf_counter() - t0
+    rate = float(off[m]) / dt / 1e6
+    return {"value": round(rate, 3), "unit": "Msamples/s", "cores": 1, "kind": "port",
+            "sample": f"first {m} of the 8192 signals ({int(off[m])} samples), universal soft "
+                      "smoothing signal by signal, single thread, oracle/wvoracle.c",
+            "full_workload_s_est": round(float(off[-1]) / (rate * 1e6), 1)}
+
+
 def batch_bytes(lens, ttypes, levels, filters, esz):
     """Words a fused smoothing pipeline must move per signal, summed: the
     DWT reads x, writes and re-reads its n coefficients, writes y (4 n);
@@ -131,10 +205,27 @@ def batch_bytes(lens, ttypes, levels, filters, esz):
     return tot * esz
 
 
-def batch_section(steps, warmup, plain_sample):
+def both_transforms(x, lens, filters, ttypes, levels):
+    """The batch with every signal twice, once per transform: DWT at full
+    depth and MODWT at J = min(8, k - 2)."""
+    import numpy as np
+
+    k = np.log2(lens).astype(np.int32)
+    off = np.concatenate([[0], np.cumsum(lens.astype(np.int64))])
+    xs = np.concatenate([np.concatenate([x[off[i]:off[i + 1]], x[off[i]:off[i + 1]]])
+                         for i in range(len(lens))])
+    l2 = np.repeat(lens, 2)
+    f2 = np.repeat(filters, 2)
+    t2 = np.tile(np.array([0, 1], dtype=np.int32), len(lens))
+    v2 = np.where(t2 == 1, np.minimum(8, np.repeat(k, 2) - 2), 0).astype(np.int32)
+    return xs, l2, f2, t2, v2
+
+
+def batch_section(steps, warmup, plain_sample, both=False):
     """Batched smoothing of the mixed 8192-signal workload (one persistent
     launch, replayed from a HIP graph) against the plain per-signal GPU
-    loop (one universal-threshold smoothing call per signal)."""
+    loop (one universal-threshold smoothing call per signal).  both=True:
+    every signal smoothed by both transforms (16384 items)."""
     import numpy as np
     import torch
 
@@ -144,9 +235,12 @@ def batch_section(steps, warmup, plain_sample):
     from wavecuda_amd.signals import batch_workload
 
     x, lens, filters, ttypes, levels = batch_workload()
+    desc = "DWT full depth or MODWT J=min(8,k-2) (one drawn per signal)"
+    if both:
+        x, lens, filters, ttypes, levels = both_transforms(x, lens, filters, ttypes, levels)
+        desc = "both DWT full depth and MODWT J=min(8,k-2) for every signal (16384 items)"
     out = {"workload": "8192 doppler+noise signals, n=2^k k~U{8..16}, filter~U{Haar,D4,C6,LA8}, "
-                       "DWT full depth or MODWT J=min(8,k-2), SNR~U[1,10]; "
-                       "forward -> universal soft threshold -> inverse",
+                       f"{desc}, SNR~U[1,10]; forward -> universal soft threshold -> inverse",
            "signals": int(len(lens)), "samples": int(lens.astype(np.int64).sum())}
     for dtype, tag in ((torch.float64, "f64"), (torch.float32, "f32")):
         esz = 8 if dtype == torch.float64 else 4
@@ -440,6 +534,33 @@ def copy_calibration(nbytes=1 << 30, reps=4):
                       "GBs_read_plus_write": round(2 * nbytes / ms / 1e6, 1)}))
 
 
+def summary(out):
+    """The headline figures again, compact, at the end of the line (a
+    reader of the line's tail sees them)."""
+    s = {"msamples_per_s": out["value"], "ms_per_step": out["ms_per_step"],
+         "step_hbm_frac": out["step_hbm_frac"], "fwd_frac": out["roofline"]["frac"],
+         "inv_frac": out["inverse_kernel"]["frac"]}
+    for key, tag in (("batched", "batch"), ("batched_both_transforms", "batch_both")):
+        b = out.get(key)
+        if b:
+            for dt in ("f64", "f32"):
+                s[f"{tag}_{dt}_ms"] = b[dt]["batched_ms"]
+                s[f"{tag}_{dt}_hbm_frac"] = b[dt]["hbm_frac"]
+                s[f"{tag}_{dt}_speedup_vs_plain"] = b[dt]["speedup_vs_plain"]
+                s[f"{tag}_{dt}_bitwise_equal"] = b[dt]["bitwise_equal_to_plain"]
+    if "host_c_abi" in out:
+        s["host_c_abi_msamples_per_s"] = out["host_c_abi"]["msamples_per_s_fwd_plus_inv"]
+    if "small" in out:
+        s["d4_n1024_us"] = out["small"]["us_per_fwd_inv"]
+    if "cpu_baseline" in out:
+        s["cpu_msamples_per_s"] = out["cpu_baseline"]["value"]
+        s["cpu_openmp_msamples_per_s"] = out["cpu_baseline_openmp"]["value"]
+        c = out["cpu_baseline_configs"]
+        s["cpu_d4_n1024_us"] = c["d4_dwt_n1024"]["value"]
+        s["cpu_batch_msamples_per_s"] = c["batch_8192"]["value"]
+    return s
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -457,6 +578,8 @@ def main():
                     help="skip the sweep, small, smoothing and differentiable sections")
     ap.add_argument("--plain-sample", type=int, default=8192,
                     help="signals of the batch run through the plain per-signal loop")
+    ap.add_argument("--cpu-batch-sample", type=int, default=1024,
+                    help="signals of the batch smoothed by the CPU oracle for its baseline")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -588,6 +711,8 @@ def main():
     }
     if not args.no_batch:
         out["batched"] = batch_section(max(3, args.steps // 5), 2, args.plain_sample)
+        out["batched_both_transforms"] = batch_section(max(3, args.steps // 5), 2,
+                                                       min(args.plain_sample, 2048), both=True)
     if not args.no_extra:
         out["sweep"] = sweep_section(max(5, args.steps // 5))
         out["small"] = small_section(max(5, args.steps // 5))
@@ -598,6 +723,12 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(J=J)
         out["cpu_baseline_openmp"] = cpu_baseline(J=J, omp=True, budget_s=5.0)
+        out["cpu_baseline_configs"] = {
+            "d4_dwt_n1024": cpu_config1(),
+            "smoothing_n2p18": cpu_config3(),
+            "batch_8192": cpu_config4(args.cpu_batch_sample),
+        }
+    out["summary"] = summary(out)
     print(json.dumps(out))
 
 

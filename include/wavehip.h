@@ -187,6 +187,39 @@ int wh_coeff_level(const void* coeffs, void* out, int64_t n, int nlevels, int tt
 int wh_coeff_labels(int64_t n, int nlevels, int ttype, int filter, int* level, double* translate,
                     unsigned char* type, void* stream);
 
+/* The data the reference's WSTtoWavethresh hands to the wavethresh R
+   package (R/rfunctions.R:864-893): it builds a zero wd object of n values
+   (wavethresh level count log2 n; type "wavelet" for a DWT, "station" for
+   a MODWT; periodic boundary), then for l = 1..nlevels calls putD at
+   wavethresh level log2(n) - l with the level-l details in translate order,
+   then putC at level log2(n) - nlevels (R's loop variable keeps its last
+   value) with the level-nlevels scaling coefficients.
+
+   wh_wavethresh_layout (host only, no device needed) fills table[0..J] in
+   that call order and returns J + 1 (negative status on error):
+   wt_level = the wavethresh level, coeff_type 0 = putD / 1 = putC, count
+   values at offset within the packed buffer.  *filter_number and family
+   receive the wd arguments (Haar 1 and D4 2 "DaubExPhase", C6 3 and LA8 4
+   "DaubLeAsymm"; the reference's switch keys LA8 as "LA", so its own call
+   leaves filter.number unset for an "LA8" object); *warnings the reference's
+   warning flags, 1: C6 (wavethresh has no coiflets, so it cannot
+   reconstruct), 2: MODWT (wavethresh orders those coefficients
+   differently).  wh_wavethresh_pack gathers every vector into `out`
+   (DEVICE, sum of the counts: n for a DWT, n (J + 1) for a MODWT) in one
+   stream-ordered pass per vector.  The wd object itself belongs to
+   wavethresh (an R package absent here): these give its inputs. */
+typedef struct wh_wt_entry {
+  int wt_level;
+  int coeff_type;
+  int64_t offset;
+  int64_t count;
+} wh_wt_entry;
+int wh_wavethresh_layout(int64_t n, int nlevels, int ttype, int filter, wh_wt_entry* table,
+                         int cap, int* filter_number, char* family, int family_cap,
+                         int* warnings);
+int wh_wavethresh_pack(const void* coeffs, void* out, int64_t n, int nlevels, int ttype,
+                       int filter, int dtype, void* stream);
+
 /* Plan tuning for the fused MODWT: times the planner's alternatives (depth
    of the first fused level group, segments per CU, a 256-thread first
    inverse group) for this length, level count, filter and element type on

@@ -21,6 +21,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #define O_FWD 1
 #define O_BWD 0
@@ -759,4 +760,21 @@ int wvo_modwt_po(double* x, double* xmod, unsigned len, int sense, unsigned nlev
   }
   free(tmp);
   return rc;
+}
+
+/* CPU-baseline timing helper (not part of the restatement): `reps` runs of
+   a forward then inverse DWT of x (copied into work first), wall-clock
+   seconds in total.  Used where one transform is too short to time from
+   Python. */
+double wvo_time_dwt_fwd_inv(const double* x, double* work, unsigned len, unsigned nlevels,
+                            int filt, int reps) {
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  for (int r = 0; r < reps; ++r) {
+    memcpy(work, x, sizeof(double) * len);
+    if (wvo_dwt(work, len, O_FWD, nlevels, filt) || wvo_dwt(work, len, O_BWD, nlevels, filt))
+      return -1.0;
+  }
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 }

@@ -51,6 +51,10 @@ int wvo_modwt(double* x, double* xmod, unsigned len, int sense, unsigned nlevels
 /* Same transforms, OpenMP over positions within a level (CPU baseline). */
 int wvo_modwt_omp(double* x, double* xmod, unsigned len, int sense, unsigned nlevels, int filt);
 int wvo_dwt_omp(double* x, unsigned len, int sense, unsigned nlevels, int filt);
+/* timing helper for the CPU baseline: reps x (copy, forward, inverse) DWT,
+   wall seconds */
+double wvo_time_dwt_fwd_inv(const double* x, double* work, unsigned len, unsigned nlevels,
+                            int filt, int reps);
 
 /* src/thresh.cpp:142-157 */
 double wvo_thresh_coef(double coef, double thresh, int hardness);

@@ -44,6 +44,8 @@ def oracle():
         L.wvo_filter_coeffs.argtypes = [_I, _DP]
         L.wvo_dwt.argtypes = [_DP, _U, _I, _U, _I]
         L.wvo_dwt_omp.argtypes = [_DP, _U, _I, _U, _I]
+        L.wvo_time_dwt_fwd_inv.restype = C.c_double
+        L.wvo_time_dwt_fwd_inv.argtypes = [_DP, _DP, _U, _U, _I, _I]
         L.wvo_modwt.argtypes = [_DP, _DP, _U, _I, _U, _I]
         L.wvo_modwt_omp.argtypes = [_DP, _DP, _U, _I, _U, _I]
         L.wvo_thresh_coef.restype = C.c_double

@@ -67,3 +67,19 @@ def test_modwt_fp32_2p26():
     xr = D.modwt_inv(w32, n, J, "D4")
     assert (torch.linalg.vector_norm(xr.double() - x.double())
             / torch.linalg.vector_norm(x.double())).item() <= 1e-5
+
+
+def test_mad_fp32_2p26_exact():
+    """FP32 selections beyond 2^22 values take three grid digits, so the
+    keys left for the finishing workgroup fit its LDS: the median is exact
+    (a full sort of the same values), the MAD (deviations formed in FP32)
+    within FP32 rounding of the FP64 one."""
+    n = 1 << 26
+    print("MAD FP32 n=2^26", flush=True)
+    v = _x(n, 5, torch.float32)
+    s = torch.sort(v.double()).values
+    med = 0.5 * (s[(n - 1) // 2] + s[n // 2])
+    dev = torch.sort((v.double() - med).abs()).values
+    mad = 1.4826 * (0.5 * (dev[(n - 1) // 2] + dev[n // 2]))
+    assert D.median(v).item() == med.item()
+    assert abs(D.mad(v).item() - mad.item()) <= 1e-6 * mad.item()

@@ -95,6 +95,19 @@ SIGNATURES = {
                            _IP]),
 }
 
+class wh_wt_entry(C.Structure):
+    _fields_ = [
+        ("wt_level", C.c_int),
+        ("coeff_type", C.c_int),
+        ("offset", C.c_int64),
+        ("count", C.c_int64),
+    ]
+
+
+SIGNATURES["wh_wavethresh_layout"] = (_I, [_I64, _I, _I, _I, C.POINTER(wh_wt_entry), _I, _IP,
+                                            C.c_char_p, _I, _IP])
+SIGNATURES["wh_wavethresh_pack"] = (_I, [_VP, _VP, _I64, _I, _I, _I, _I, _VP])
+
 _lib = None
 
 

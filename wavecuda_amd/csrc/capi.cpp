@@ -332,8 +332,11 @@ int wh_mad(const void* x, int64_t count, int dtype, double* out, void* stream) {
 
 int wh_univ_thresh(const void* coeffs, int64_t n, int nlevels, int ttype, int filter, int dtype,
                    int minlevel, int maxlevel, double* out, void* stream) {
-  int J, rc = check_common(coeffs, out, n, nlevels, filter, dtype, &J);
+  // the coefficients are read as element pairs; out is one double
+  int J, rc = check_common(coeffs, coeffs, n, nlevels, filter, dtype, &J);
   if (rc) return rc;
+  if (!out || (reinterpret_cast<uintptr_t>(out) & 7))
+    return set_error(WH_ERR_ARG, "out must be a non-null, 8-byte aligned device double");
   if (ttype != kDWT && ttype != kMODWT) return set_error(WH_ERR_ARG, "ttype must be 0 or 1");
   if (minlevel < 0 || minlevel > maxlevel || maxlevel >= J)
     return set_error(WH_ERR_ARG, "levels must satisfy 0 <= minlevel <= maxlevel < nlevels");
@@ -397,6 +400,48 @@ int wh_coeff_labels(int64_t n, int nlevels, int ttype, int filter, int* level, d
   if (J <= 0) return set_error(WH_ERR_ARG, "length must be a power of two and nlevels <= floor(log2(n/L))+1");
   if (!level && !translate && !type) return WH_OK;
   return level_labels(n, J, ttype, level, translate, type, static_cast<hipStream_t>(stream));
+}
+
+int wh_wavethresh_layout(int64_t n, int nlevels, int ttype, int filter, wh_wt_entry* table,
+                         int cap, int* filter_number, char* family, int family_cap,
+                         int* warnings) {
+  if (!valid_filter(filter)) return -set_error(WH_ERR_ARG, "filter must be 1 (Haar), 2 (D4), 3 (C6) or 4 (LA8)");
+  if (ttype != kDWT && ttype != kMODWT) return -set_error(WH_ERR_ARG, "ttype must be 0 (DWT) or 1 (MODWT)");
+  const int J = resolve_levels(n, nlevels, filter_length(filter));
+  if (J <= 0) return -set_error(WH_ERR_ARG, "length must be a power of two and nlevels <= floor(log2(n/L))+1");
+  if (!table || cap < J + 1) return -set_error(WH_ERR_ARG, "wavethresh layout: table needs nlevels + 1 entries");
+  int lg = 0;
+  while ((int64_t(1) << lg) < n) ++lg;  // wavethresh's nlevels for a wd of length n
+  int64_t off = 0;
+  for (int e = 0; e <= J; ++e) {
+    const int l = e < J ? e + 1 : J;  // R's loop variable keeps its last value for putC
+    wh_wt_entry& t = table[e];
+    t.wt_level = lg - l;
+    t.coeff_type = e < J ? 0 : 1;
+    t.count = (ttype == kDWT) ? (n >> l) : n;
+    t.offset = off;
+    off += t.count;
+  }
+  // switch(XW$filt, "Haar"=1, "D4"=2, "C6"=3, "LA"=4) and the family
+  // switch (R/rfunctions.R:866-867)
+  static const char* fam[5] = {"", "DaubExPhase", "DaubExPhase", "DaubLeAsymm", "DaubLeAsymm"};
+  if (filter_number) *filter_number = filter;
+  if (family && family_cap > 0) snprintf(family, size_t(family_cap), "%s", fam[filter]);
+  // the reference's warnings (:869-873): 1 = C6 (no coiflets in
+  // wavethresh), 2 = MODWT (wavethresh orders it differently)
+  if (warnings) *warnings = (filter == kC6 ? 1 : 0) | (ttype == kMODWT ? 2 : 0);
+  return J + 1;
+}
+
+int wh_wavethresh_pack(const void* coeffs, void* out, int64_t n, int nlevels, int ttype,
+                       int filter, int dtype, void* stream) {
+  if (!coeffs || !out) return set_error(WH_ERR_ARG, "null pointer");
+  if (!valid_filter(filter)) return set_error(WH_ERR_ARG, "filter must be 1 (Haar), 2 (D4), 3 (C6) or 4 (LA8)");
+  if (!valid_dtype(dtype)) return set_error(WH_ERR_ARG, "dtype must be WH_F64 or WH_F32");
+  if (ttype != kDWT && ttype != kMODWT) return set_error(WH_ERR_ARG, "ttype must be 0 (DWT) or 1 (MODWT)");
+  const int J = resolve_levels(n, nlevels, filter_length(filter));
+  if (J <= 0) return set_error(WH_ERR_ARG, "length must be a power of two and nlevels <= floor(log2(n/L))+1");
+  return wavethresh_pack(coeffs, out, n, J, ttype, dtype, static_cast<hipStream_t>(stream));
 }
 
 int wh_modwt_tune(int64_t n, int nlevels, int filter, int dtype, void* stream) {

@@ -377,13 +377,18 @@ struct Pass {
   int64_t N;
 };
 
-// Largest compact length finished by one workgroup: (N + N/2) values
-// must fit the LDS budget.
+// Largest compact length finished by one workgroup: (N + N/2) values must
+// fit the LDS budget, and the one workgroup's serial walk must stay short
+// (a tile pass of at most two workgroups is cheaper than one workgroup over
+// more than kWholeWork values: measured at LA8, n = 2^24, FP32, the one
+// workgroup took 49.6 us of the inverse over 8192 values).
+constexpr int64_t kWholeWork = 2048;
+
 template <typename T>
 int64_t whole_max() {
   const size_t budget = std::min<size_t>(device_info().max_lds_per_block, 160 * 1024);
   int64_t m = 1;
-  while (m * 2 <= 16384 && size_t(m * 2 + m) * sizeof(T) <= budget) m *= 2;
+  while (m * 2 <= kWholeWork && size_t(m * 2 + m) * sizeof(T) <= budget) m *= 2;
   return m;
 }
 

@@ -6,6 +6,8 @@
 //    scaling coefficients at m 2^J) and, for a time-ordered MODWT, the
 //    level's S_l or D_l block (xmod[2n(l-1) + i], xmod[2n(l-1) + n + i],
 //    src/daub4.cpp:525-528);
+//  * the packed per-level vectors the reference's WSTtoWavethresh hands to
+//    the wavethresh package (R/rfunctions.R:864-893);
 //  * the label columns the reference's WSTtoDT attaches to every
 //    coefficient (R/rfunctions.R:925-1004) and label.detail.scaling
 //    (:902-912): coefficient type, level and translate, one entry per
@@ -95,6 +97,25 @@ int level_extract(const void* c, void* out, int64_t n, int J, int ttype, int lev
                        static_cast<const double*>(c), static_cast<double*>(out), start, gap,
                        count);
   WH_HIP_CHECK(hipGetLastError());
+  return WH_OK;
+}
+
+// WSTtoWavethresh (R/rfunctions.R:864-893): the vectors its loop hands to
+// wavethresh's putD for l = 1..J (level log2(n) - l) and then to putC (the
+// level-J scaling coefficients), packed back to back in that order
+int wavethresh_pack(const void* c, void* out, int64_t n, int J, int ttype, int dtype,
+                    hipStream_t st) {
+  const size_t esz = dtype == WH_F32 ? 4 : 8;
+  int64_t off = 0;
+  for (int e = 1; e <= J + 1; ++e) {
+    const int level = e <= J ? e : J;
+    const int coeff_s = e <= J ? 0 : 1;
+    int64_t cnt = 0;
+    int rc = level_extract(c, static_cast<char*>(out) + size_t(off) * esz, n, J, ttype, level,
+                           coeff_s, dtype, st, &cnt);
+    if (rc) return rc;
+    off += cnt;
+  }
   return WH_OK;
 }
 

@@ -493,12 +493,14 @@ __global__ __launch_bounds__(kFinNT) void sel_finish_kernel(SelSrc s, SelState* 
 // Number of 11-bit digits resolved by grid passes before the keys still
 // in play are compacted.  After two digits the bucket of a middle rank
 // holds the values within about 2^-13 (FP32 keys) or 2^-10 (FP64 keys) of
-// it; FP64 keys take a third grid digit beyond 2^22 values, so the bucket
-// (2^-21) still fits the finishing workgroup's LDS.  Beyond 2^31 values
-// every digit is a grid pass (the finishing workgroup counts in 32 bits).
+// it; beyond 2^22 values (either type) a third grid digit narrows it, so
+// the keys left (about 2^22 * 2^-13 of a Gaussian-like sample for FP32)
+// still fit the finishing workgroup's LDS instead of overflowing into its
+// slow path that reads every value.  Beyond 2^31 values every digit is a
+// grid pass (the finishing workgroup counts in 32 bits).
 inline int grid_digits(int key_bits, int64_t E) {
   if (E >= (int64_t(1) << 31)) return (key_bits + kBits - 1) / kBits;
-  return (key_bits == 32 || E <= (int64_t(1) << 22)) ? 2 : 3;
+  return E <= (int64_t(1) << 22) ? 2 : 3;
 }
 
 // Grid histogram passes over the leading digits, then a compaction of the

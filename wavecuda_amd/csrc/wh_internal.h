@@ -92,6 +92,9 @@ int level_extract(const void* c, void* out, int64_t n, int J, int ttype, int lev
                   int dtype, hipStream_t st, int64_t* count_out);
 int level_labels(int64_t n, int J, int ttype, int* level, double* translate, unsigned char* type,
                  hipStream_t st);
+// WSTtoWavethresh's putD / putC vectors packed in its loop order
+int wavethresh_pack(const void* c, void* out, int64_t n, int J, int ttype, int dtype,
+                    hipStream_t st);
 
 // modwt_order.hip: every level of a 2nJ MODWT vector between time order and
 // packet order (to_po: time ordered in), out of place

@@ -351,3 +351,58 @@ def WSTtoDT(Xwav: WST, scaling=True, forPlotting=False):
     if not scaling:
         df = df[df["CoeffType"] == "d"].reset_index(drop=True)
     return df
+
+
+# ---- wavethresh interop: WSTtoWavethresh -----------------------------------
+
+def wavethresh_layout(n, nlevels, ttype, filt):
+    """The calls WSTtoWavethresh (R/rfunctions.R:864-893) makes on a zero
+    wavethresh wd object, from wh_wavethresh_layout: a dict with the wd
+    arguments (filter.number, family, type, bc), the warning flags, and the
+    putD / putC calls in order as (wavethresh level, "d" or "s", offset,
+    count) into the packed vector of wh_wavethresh_pack."""
+    cap = 64
+    table = (W.wh_wt_entry * cap)()
+    fnum = C.c_int(0)
+    fam = C.create_string_buffer(32)
+    warn = C.c_int(0)
+    k = W.lib().wh_wavethresh_layout(int(n), int(nlevels), _TTYPE[ttype], _FILT[filt], table,
+                                     cap, C.byref(fnum), fam, 32, C.byref(warn))
+    if k <= 0:
+        raise ValueError(W.lib().wh_last_error().decode())
+    calls = [(table[i].wt_level, "s" if table[i].coeff_type else "d", table[i].offset,
+              table[i].count) for i in range(k)]
+    return {"filter.number": fnum.value, "family": fam.value.decode(),
+            "type": "wavelet" if ttype == "DWT" else "station", "bc": "periodic",
+            "warnings": warn.value, "calls": calls}
+
+
+def WSTtoWavethresh(XW: WST, showWarnings=True):
+    """WSTtoWavethresh (R/rfunctions.R:864-893): what it hands to wavethresh
+    -- the wd() arguments and, in call order, each putD(level, v) and the
+    final putC(level, v) -- with v gathered on the GPU by
+    wh_wavethresh_pack.  (The wd object itself is built by the wavethresh R
+    package, which is not part of this library.)  The reference's warnings
+    are raised as UserWarning."""
+    import warnings
+
+    import torch
+
+    lay = wavethresh_layout(XW.len, XW.nlevels, XW.ttype, XW.filt)
+    if showWarnings and (lay["warnings"] & 1):
+        warnings.warn("Coiflets not implemented in wavethresh; you will not be able reconstruct "
+                      "with wavethresh")
+    if showWarnings and (lay["warnings"] & 2):
+        warnings.warn("Wavethresh uses a different ordering in MODWT coefficients")
+    n, J = XW.len, XW.nlevels
+    m = 2 * n * J if XW.ttype == "MODWT" else n
+    w = torch.tensor(np.asarray(XW.w[:m], dtype=np.float64), device="cuda")
+    total = sum(c[3] for c in lay["calls"])
+    out = torch.empty(total, dtype=torch.float64, device="cuda")
+    W.check(W.lib().wh_wavethresh_pack(w.data_ptr(), out.data_ptr(), n, J, _TTYPE[XW.ttype],
+                                       _FILT[XW.filt], W.WH_F64, None), "wh_wavethresh_pack")
+    v = out.cpu().numpy()
+    puts = [{"fn": "putD" if t == "d" else "putC", "level": lev, "v": v[o:o + c]}
+            for lev, t, o, c in lay["calls"]]
+    return {"wd": {"n": n, "filter.number": lay["filter.number"], "family": lay["family"],
+                   "type": lay["type"], "bc": lay["bc"]}, "puts": puts}
