@@ -42,7 +42,8 @@ __device__ __forceinline__ double sel_val(uint64_t k, float*) {
 }
 
 // LDS layout of block_median: the candidate keys (8-byte aligned) first,
-// then the two histograms, the scan array and the selection state.
+// then the two histograms, the scan array (the wave totals of the bucket
+// scan) and the selection state.
 template <int NT>
 constexpr size_t block_sel_lds() {
   return size_t(kSelCap) * 8 + size_t(2) * kSelBins * 4 + size_t(NT) * 4 + 128;
@@ -115,24 +116,41 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
       sel_for_each<T, NT>(v, off, stride, count, deviation, c, count_key);
     }
     __syncthreads();
+    // Both ranks' bucket counts packed in one word (rank 0 in the low 32
+    // bits, rank 1 in the high; each is at most count < 2^32, so the low
+    // half never carries), so one scan serves both: each thread sums its
+    // `per` buckets, a wave scans by shuffles, then the wave totals.
     constexpr int per = kSelBins / NT;
-    for (int r = 0; r < 2; ++r) {
-      const unsigned* hr = hist + (same ? 0 : r) * kSelBins;
-      const long long k = ks[r];  // read before the barriers; the finder updates it below
-      unsigned loc = 0;
+    constexpr int kWaves = NT / 64;
+    uint64_t* wsum = reinterpret_cast<uint64_t*>(part);  // [kWaves]
+    const long long k0 = ks[0], k1 = ks[1];  // the finder below updates ks
+    const unsigned* h1 = hist + (same ? 0 : kSelBins);
+    uint64_t loc = 0;
 #pragma unroll
-      for (int i = 0; i < per; ++i) loc += hr[tid * per + i];
-      part[tid] = loc;
-      __syncthreads();
-      for (int o = 1; o < NT; o <<= 1) {
-        unsigned add = (tid >= o) ? part[tid - o] : 0u;
-        __syncthreads();
-        part[tid] += add;
-        __syncthreads();
-      }
-      const long long before = (tid == 0) ? 0 : (long long)part[tid - 1];
-      if (before <= k && k < (long long)part[tid]) {
-        long long acc = before;
+    for (int i = 0; i < per; ++i)
+      loc += uint64_t(hist[tid * per + i]) | (uint64_t(h1[tid * per + i]) << 32);
+    uint64_t inc = loc;
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint64_t t = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += t;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint64_t base = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w)
+      if (w < wave) base += wsum[w];
+    const uint64_t before = base + inc - loc;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const unsigned* hr = r ? h1 : hist;
+      const long long k = r ? k1 : k0;
+      const long long bef = (long long)(r ? (before >> 32) : (before & 0xffffffffull));
+      const long long mine = (long long)(r ? (loc >> 32) : (loc & 0xffffffffull));
+      if (bef <= k && k < bef + mine) {
+        long long acc = bef;
         for (int i = 0; i < per; ++i) {
           const unsigned cnt = hr[tid * per + i];
           if (acc + cnt > k) {
@@ -144,10 +162,13 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
           acc += cnt;
         }
       }
-      __syncthreads();
-      if (tid == 0) st[r] = (st[r] << (hi - lo)) | st[2 + r];
-      __syncthreads();
     }
+    __syncthreads();
+    if (tid == 0) {
+      st[0] = (st[0] << (hi - lo)) | st[2];
+      st[1] = (st[1] << (hi - lo)) | st[3];
+    }
+    __syncthreads();
     hi = lo;
     if (!in_lds && hi > 0) {
       const uint64_t q0 = st[0], q1 = st[1];
