@@ -11,7 +11,11 @@
 // any, so a pass costs a few memory latencies rather than one per element.
 // As soon as the histogram shows that the values still matching the
 // selected prefixes fit in LDS (kSelCap keys), one more pass copies their
-// keys there and the remaining digits are resolved in LDS.
+// keys there and the remaining digits are resolved in LDS -- or, when at
+// most kSelDirect keys are left (the usual case: a middle rank's bucket
+// holds a few dozen values after one or two digits), every candidate's rank
+// within its bucket is counted directly, one pass and one barrier instead
+// of a radix digit's five barriers for each remaining digit.
 #pragma once
 
 #include "wh_common.h"
@@ -22,6 +26,7 @@ constexpr int kSelBits = 11;
 constexpr int kSelBins = 1 << kSelBits;
 constexpr int kSelCap = 2048;   // keys kept in LDS
 constexpr int kSelUnroll = 8;   // loads in flight per thread and pass
+constexpr int kSelDirect = 256; // candidates ranked by direct counting
 
 __device__ __forceinline__ uint64_t sel_key(double v) {
   uint64_t u = __double_as_longlong(v);
@@ -98,6 +103,32 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
   bool in_lds = false;
   int nc = 0;
   for (int hi = kbits; hi > 0;) {
+    if (in_lds && nc <= kSelDirect) {
+      // rank of each candidate among the candidates of its bucket: the key
+      // of rank k in bucket q is the x with lt <= k < lt + eq
+      const uint64_t q0 = st[0], q1 = st[1];
+      const long long k0 = ks[0], k1 = ks[1];
+      __syncthreads();  // every thread has read st and ks before they change
+      for (int i = tid; i < nc; i += NT) {
+        const uint64_t x = cand[i], tx = x >> hi;
+        long long lt0 = 0, eq0 = 0, lt1 = 0, eq1 = 0;
+        for (int j = 0; j < nc; ++j) {
+          const uint64_t y = cand[j], ty = y >> hi;
+          if (ty == q0) {
+            lt0 += (y < x);
+            eq0 += (y == x);
+          }
+          if (ty == q1) {
+            lt1 += (y < x);
+            eq1 += (y == x);
+          }
+        }
+        if (tx == q0 && lt0 <= k0 && k0 < lt0 + eq0) st[0] = x;
+        if (tx == q1 && lt1 <= k1 && k1 < lt1 + eq1) st[1] = x;
+      }
+      __syncthreads();
+      break;
+    }
     const int lo = hi > kSelBits ? hi - kSelBits : 0;
     for (int i = tid; i < 2 * kSelBins; i += NT) hist[i] = 0;
     __syncthreads();
