@@ -581,7 +581,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--log2n", type=int, default=24)
     ap.add_argument("--levels", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
@@ -631,29 +631,35 @@ def main():
         D.modwt_fwd(x, J, filt, out=xmod)
         D.modwt_inv(xmod, n, J, filt, out=xr)
 
-    for _ in range(args.warmup):
-        step()
+    # the reconstruction check runs before the warm-up: the card idles for
+    # tens of ms while it runs (first calls of torch kernels), and after such
+    # a pause the inverse runs up to 25 % slower for some 20 steps, so the
+    # warm-up steps lead straight into the timed ones
+    step()
     torch.cuda.synchronize()
     err = ((xr - x).abs().max() / x.abs().max()).item()
     if not err <= 1e-12:
         print(f"reconstruction error {err} exceeds 1e-12", file=sys.stderr)
         sys.exit(1)
 
-    # timed region: K steps, events around each kernel on the current stream
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
-           torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    # timed region: K steps, an event at every kernel boundary on the current
+    # stream (one event ends step k and starts step k + 1)
+    bounds = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    mids = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    for _ in range(args.warmup):
+        step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for e0, e1, e2 in ev:
-        e0.record()
+    bounds[0].record()
+    for k in range(args.steps):
         D.modwt_fwd(x, J, filt, out=xmod)
-        e1.record()
+        mids[k].record()
         D.modwt_inv(xmod, n, J, filt, out=xr)
-        e2.record()
+        bounds[k + 1].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    fwd_ms = sum(a.elapsed_time(b) for a, b, _ in ev) / args.steps
-    inv_ms = sum(b.elapsed_time(c) for _, b, c in ev) / args.steps
+    fwd_ms = sum(bounds[k].elapsed_time(mids[k]) for k in range(args.steps)) / args.steps
+    inv_ms = sum(mids[k].elapsed_time(bounds[k + 1]) for k in range(args.steps)) / args.steps
     ms_per_step = wall * 1e3 / args.steps
 
     # the same step with a 1 GiB write between steps (outside the timed

@@ -205,6 +205,20 @@ __global__ void zero_slot_k(T* __restrict__ p, int64_t n, int64_t stride) {
     q[i] = T(0);
 }
 
+// the tuner's test signal: values uniform in [-1, 1) from a hash of the
+// index (a zero signal runs at other speeds than real data does)
+template <typename T>
+__global__ void tune_signal_k(T* __restrict__ p, int64_t n) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    uint64_t h = uint64_t(i) * 0x9e3779b97f4a7c15ull;
+    h ^= h >> 29;
+    h *= 0xbf58476d1ce4e5b9ull;
+    h ^= h >> 32;
+    p[i] = T(double(h >> 11) * 0x1.0p-52 - 1.0);
+  }
+}
+
 template <typename T>
 int modwt_fwd_t(const void* x, void* xmod, int64_t n, int64_t count, int J, int filt, int sp,
                 hipStream_t st) {
@@ -362,9 +376,11 @@ int tune_t(int64_t n, int J, int filt, hipStream_t st) {
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = WH_OK;
-  if (hipMemsetAsync(x, 0, size_t(n) * esz, st) != hipSuccess ||
-      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
-    rc = set_hip_error(hipGetLastError(), "modwt tune: setup");
+  hipLaunchKernelGGL(tune_signal_k<T>, dim3(1024), dim3(256), 0, st, static_cast<T*>(x), n);
+  hipError_t he = hipGetLastError();
+  if (he == hipSuccess) he = hipEventCreate(&e0);
+  if (he == hipSuccess) he = hipEventCreate(&e1);
+  if (he != hipSuccess) rc = set_hip_error(he, "modwt tune: setup");
   for (int inv = 0; inv < 2 && !rc; ++inv) {
     const bool inverse = inv == 1;
     std::map<std::string, PlanKnobs> cands;  // distinct plans only
