@@ -35,6 +35,7 @@ constexpr int kTile = 2048;   // coefficients per tile (level j0 units)
 constexpr int kNT = 256;      // threads per tile workgroup
 constexpr int kNTW = 1024;    // threads of the whole-sequence workgroup
 constexpr int kGmax = 5;      // levels per tile pass
+constexpr int kWholeWork = 2048;  // longest sequence a "whole" kernel finishes
 
 __host__ __device__ constexpr int halo_fwd(int levels, int L) { return ((1 << levels) - 1) * (L - 2); }
 
@@ -74,6 +75,7 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
   // barrier, so one buffer serves every level (four workgroups per CU)
   constexpr int kPer = (kTile / 2 + halo_fwd(kGmax - 1, L) + kNT - 1) / kNT;
   using V = typename Vec2<T>::type;
+  typedef T E2 __attribute__((ext_vector_type(2)));
   __shared__ __align__(16) T bufA[SZA + 2];
   __shared__ __align__(16) T dbuf[kTile];
 
@@ -85,8 +87,31 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
 
   const int64_t base = int64_t(blockIdx.x) * kTile;
   const int K0 = kTile + halo_fwd(g, L);
-  for (int q = 2 * tid; q < K0; q += 2 * kNT)
-    *reinterpret_cast<V*>(bufA + q) = *reinterpret_cast<const V*>(src + ((base + q) & (N - 1)));
+  // every load of the tile is issued before the first one is used (a loop
+  // that stores each load to LDS before issuing the next waits one memory
+  // latency per iteration); only the last tile's halo wraps around
+  constexpr int kLd = (SZA / 2 + kNT - 1) / kNT;
+  auto stage = [&](auto addr) {
+    E2 ld[kLd];  // (an array of HIP's double2 / float2 structs would live in scratch)
+#pragma unroll
+    for (int k = 0; k < kLd; ++k) {
+      const int q = 2 * (tid + k * kNT);
+      // unconditional (a clamped duplicate past the end): a guarded load
+      // would be merged with its guarded store and wait there
+      ld[k] = *reinterpret_cast<const E2*>(addr(q < K0 ? q : K0 - 2));
+    }
+#pragma unroll
+    for (int k = 0; k < kLd; ++k) {
+      const int q = 2 * (tid + k * kNT);
+      if (q < K0) *reinterpret_cast<E2*>(bufA + q) = ld[k];
+    }
+  };
+  if (base + K0 <= N) {
+    const T* sp = src + base;
+    stage([&](int q) { return sp + q; });
+  } else {
+    stage([&](int q) { return src + ((base + q) & (N - 1)); });
+  }
   __syncthreads();
 
   T* cur = bufA;
@@ -162,6 +187,7 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
   // below it (the lowest detail any of its levels reads)
   constexpr int HY = (L - 2) << kGmax;
   using V = typename Vec2<T>::type;
+  typedef T E2 __attribute__((ext_vector_type(2)));
   // the pool holds the staged y span, then s of the current level: each
   // level's outputs go through registers (at most kPairs pairs a thread)
   // and back over its input after a barrier, so one buffer serves every
@@ -188,58 +214,112 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
     for (int i = 1; i <= l; ++i) ee = ((ee / 2 + HP) + 1) & ~1;
     return ee;
   };
-  // details d_l over local [0, own_l + e_l), global m = B_l - e_l + i
-  // The first pass (j0 = 0) reads its y span once, with contiguous 16-byte
-  // loads, into the pool and takes the strided details from there; deeper
-  // passes read their (few) details from y directly.
-  const bool staged = (j0 == 0) && (base >= HY) && (N >= kTile + HY);
-  if (staged) {
-    const T* ys = y + (base - HY);
-    for (int i = tid; i < (kTile + HY) / 2; i += kNT)
-      *reinterpret_cast<V*>(pool + 2 * i) = *reinterpret_cast<const V*>(ys + 2 * i);
-    __syncthreads();
+  // Every pass stages the span [base - HY, base + kTile) of the level-j0
+  // sequence (y at stride 2^j0, positions mod N) in the pool: it holds
+  // every detail the tile's g levels read.  The first pass (j0 = 0) reads
+  // it with contiguous 16-byte loads; deeper passes with one load per
+  // value.  All these loads, and those of the top s_{j0+g}, are issued
+  // before any is used, so the staging costs one memory latency.
+  constexpr int kSpan = kTile + HY;
+  constexpr int kLdP = (kSpan / 2 + kNT - 1) / kNT;  // pair loads a thread (j0 = 0)
+  constexpr int kLdS = (kSpan + kNT - 1) / kNT;      // single loads a thread (j0 > 0)
+  constexpr int kTopMax = (kTile / 2 + L + kNT - 1) / kNT;
+  const int span = int(std::min<int64_t>(kSpan, N));  // pool positions used
+  const int64_t sbase = base - HY;                    // compact position of pool[0]
+  const int own_g = kTile >> g;
+  const int eg = evn(g);
+  const int cnt_g = own_g + eg;
+  const int64_t Bg = base >> g;
+  const int64_t Ng = N >> g;
+  // top s_{j0+g}: one load a thread unless the pass has fewer than kGmax
+  // levels (a uniform branch); loads past cnt_g are clamped duplicates
+  T topv[kTopMax];
+  auto load_top = [&] {
+    topv[0] = src[((Bg - eg + (tid < cnt_g ? tid : cnt_g - 1)) & (Ng - 1)) << a.src_shift];
+    if (cnt_g > kNT) {
+#pragma unroll
+      for (int k = 1; k < kTopMax; ++k) {
+        const int i = tid + k * kNT;
+        const int ic = i < cnt_g ? i : cnt_g - 1;
+        topv[k] = src[((Bg - eg + ic) & (Ng - 1)) << a.src_shift];
+      }
+    }
+  };
+  // Only the first tile's halo (or a sequence shorter than the span) wraps
+  // around; every other tile addresses its span from one base pointer.
+  const bool wrap = (base < HY) || (N < kSpan);
+  if (j0 == 0) {
+    E2 ld[kLdP];
+    if (!wrap) {
+      const T* ys = y + sbase;
+#pragma unroll
+      for (int k = 0; k < kLdP; ++k) {
+        const int i = 2 * (tid + k * kNT);
+        ld[k] = *reinterpret_cast<const E2*>(ys + (i < span ? i : span - 2));  // clamped
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kLdP; ++k) {
+        const int i = 2 * (tid + k * kNT);
+        const int ic = i < span ? i : span - 2;  // unconditional load, clamped
+        ld[k] = *reinterpret_cast<const E2*>(y + ((sbase + ic) & (N - 1)));
+      }
+    }
+    load_top();
+#pragma unroll
+    for (int k = 0; k < kLdP; ++k) {
+      const int i = 2 * (tid + k * kNT);
+      if (i < span) *reinterpret_cast<E2*>(pool + i) = ld[k];
+    }
+  } else {
+    T ld[kLdS];
+    if (!wrap) {
+      const T* ys = y + (sbase << j0);
+#pragma unroll
+      for (int k = 0; k < kLdS; ++k) {
+        const int i = tid + k * kNT;
+        ld[k] = ys[uint64_t(i < span ? i : span - 1) << j0];  // clamped
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kLdS; ++k) {
+        const int i = tid + k * kNT;
+        const int ic = i < span ? i : span - 1;  // unconditional load, clamped
+        ld[k] = y[((sbase + ic) & (N - 1)) << j0];
+      }
+    }
+    load_top();
+#pragma unroll
+    for (int k = 0; k < kLdS; ++k) {
+      const int i = tid + k * kNT;
+      if (i < span) pool[i] = ld[k];
+    }
   }
+  __syncthreads();
+  // details d_l over local [0, own_l + e_l): level-l index m = B_l - e_l + i
+  // at compact position (2m + 1) 2^{l-1}, pool slot (that - sbase) mod N =
+  // ((2i + 1) 2^{l-1} + HY - e_l 2^l) mod N (B_l 2^l = base)
+  const uint32_t pmask = N > int64_t(0xffffffff) ? 0xffffffffu : uint32_t(N - 1);
   int doff = 0;
   for (int l = 1; l <= g; ++l) {
     const int own = kTile >> l;
     const int el = evn(l);
     const int cnt = own + el;
-    const int64_t Nl = N >> l;
-    const int64_t Bl = base >> l;
-    const int jg = j0 + l;  // global 1-based level
-    const T t = thr_level<T>(a.thr, jg - 1);
-    // two loops, so the staged one reads LDS with LDS instructions (one
-    // loop with a select between the two sources compiles to flat loads)
-    if (staged) {
-      for (int i = tid; i < cnt; i += kNT) {
-        const int64_t m = (Bl - el + i) & (Nl - 1);
-        T d = pool[((2 * m + 1) << (jg - 1)) - (base - HY)];
-        if (t >= T(0)) d = thresh_coef(d, t, a.thr.hardness);
-        dbuf[doff + i] = d;
-      }
-    } else {
-      for (int i = tid; i < cnt; i += kNT) {
-        const int64_t m = (Bl - el + i) & (Nl - 1);
-        T d = y[(2 * m + 1) << (jg - 1)];
-        if (t >= T(0)) d = thresh_coef(d, t, a.thr.hardness);
-        dbuf[doff + i] = d;
-      }
+    const uint32_t cl = uint32_t(HY - (el << l));
+    const T t = thr_level<T>(a.thr, j0 + l - 1);
+    for (int i = tid; i < cnt; i += kNT) {
+      T d = pool[((uint32_t(2 * i + 1) << (l - 1)) + cl) & pmask];
+      if (t >= T(0)) d = thresh_coef(d, t, a.thr.hardness);
+      dbuf[doff + i] = d;
     }
     doff += own + PADS;
   }
-  if (staged) __syncthreads();  // the pool is reused for s below
+  __syncthreads();  // the pool is reused for s below
   // top s_{j0+g}, same range as d_g
-  {
-    const int own = kTile >> g;
-    const int eg = evn(g);
-    const int cnt = own + eg;
-    const int64_t Ng = N >> g;
-    const int64_t Bg = base >> g;
-    T* top = bufS;
-    for (int i = tid; i < cnt; i += kNT) {
-      int64_t m = (Bg - eg + i) & (Ng - 1);
-      top[i] = src[m << a.src_shift];
-    }
+#pragma unroll
+  for (int k = 0; k < kTopMax; ++k) {
+    const int i = tid + k * kNT;
+    if (i < cnt_g) bufS[i] = topv[k];
   }
   __syncthreads();
   // levels g..1: level l reads s_l, d_l (start B_l - e_l) and writes
@@ -329,34 +409,59 @@ __global__ __launch_bounds__(kNTW) void dwt_fwd_whole(WholeArgs a) {
 template <typename T, int FILT>
 __global__ __launch_bounds__(kNTW) void dwt_inv_whole(WholeArgs a) {
   constexpr int L = Filter<FILT>::L;
+  constexpr int kLd = (kWholeWork + kNTW - 1) / kNTW;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* A = reinterpret_cast<T*>(smem_raw);
   T* B = A + a.N;
+  T* Cf = B + a.N / 2;  // the N coefficients of levels j0+1..J, thresholded
   const int tid = threadIdx.x;
   const T* __restrict__ y = static_cast<const T*>(a.src);
   T* __restrict__ dst = static_cast<T*>(a.dst);
   const int N = int(a.N);
   const int j0 = a.j0, J = a.J;
+  // The level-j0 sequence y[q 2^j0], q < N, holds every coefficient the
+  // remaining levels read: s_J where q is a multiple of 2^{J-j0}, else the
+  // detail of level j0 + ctz(q) + 1.  It is loaded once, every load issued
+  // before the first is used, and thresholded as it goes to LDS.
+  T ld[kLd];
+#pragma unroll
+  for (int k = 0; k < kLd; ++k) {
+    const int q = tid + k * kNTW;
+    const int qc = q < N ? q : N - 1;  // unconditional load, clamped
+    ld[k] = y[int64_t(qc) << j0];
+  }
+#pragma unroll
+  for (int k = 0; k < kLd; ++k) {
+    const int q = tid + k * kNTW;
+    if (q < N) {
+      const int z = (q == 0) ? 31 : __builtin_ctz(unsigned(q));
+      T d = ld[k];
+      if (z < J - j0) {
+        const T t = thr_level<T>(a.thr, j0 + z);
+        if (t >= T(0)) d = thresh_coef(d, t, a.thr.hardness);
+      }
+      Cf[q] = d;
+    }
+  }
+  __syncthreads();
   // s_J into the buffer that level J reads: output of level j goes to A
   // when (j - j0) is odd, B otherwise; level J reads the output slot of J+1
   int NJ = N >> (J - j0);
   T* top = ((J + 1 - j0) & 1) ? A : B;
-  for (int m = tid; m < NJ; m += kNTW) top[m] = y[int64_t(m) << J];
+  for (int m = tid; m < NJ; m += kNTW) top[m] = Cf[m << (J - j0)];
   __syncthreads();
   int Nl = NJ;  // length of s_j
   for (int j = J; j > j0; --j) {
     const T* cur = ((j + 1 - j0) & 1) ? A : B;
     T* out = ((j - j0) & 1) ? A : B;
-    const T t = thr_level<T>(a.thr, j - 1);
+    const int sh = j - 1 - j0;
     for (int m = tid; m < Nl; m += kNTW) {
       T sv[L / 2], dv[L / 2];
 #pragma unroll
       for (int q = 0; q < L / 2; ++q) {
         int mm = (m - q) & (Nl - 1);
         sv[q] = cur[mm];
-        T d = y[int64_t(2 * mm + 1) << (j - 1)];
-        if (t >= T(0)) d = thresh_coef(d, t, a.thr.hardness);
-        dv[q] = d;
+        dv[q] = Cf[(2 * mm + 1) << sh];
       }
       T ev, ov;
       inv_pair<T, FILT>(sv, dv, ev, ov);
@@ -377,18 +482,16 @@ struct Pass {
   int64_t N;
 };
 
-// Largest compact length finished by one workgroup: (N + N/2) values must
-// fit the LDS budget, and the one workgroup's serial walk must stay short
-// (a tile pass of at most two workgroups is cheaper than one workgroup over
-// more than kWholeWork values: measured at LA8, n = 2^24, FP32, the one
-// workgroup took 49.6 us of the inverse over 8192 values).
-constexpr int64_t kWholeWork = 2048;
+// Largest compact length finished by one workgroup: its LDS (2.5 N values
+// for the inverse) must fit the budget, and the one workgroup's serial walk
+// must stay short (kWholeWork, with kTile: a tile pass of one or two
+// workgroups is cheaper than one workgroup over more values).
 
 template <typename T>
 int64_t whole_max() {
   const size_t budget = std::min<size_t>(device_info().max_lds_per_block, 160 * 1024);
   int64_t m = 1;
-  while (m * 2 <= kWholeWork && size_t(m * 2 + m) * sizeof(T) <= budget) m *= 2;
+  while (m * 2 <= kWholeWork && size_t(5 * m) * sizeof(T) <= budget) m *= 2;
   return m;
 }
 
@@ -516,7 +619,7 @@ int dwt_inv_f(const void* yin, void* x, int64_t n, int J, const wh_thresh* thr,
       a.j0 = P.j0;
       a.J = J;
       a.thr = th;
-      size_t lds = size_t(P.N + P.N / 2) * sizeof(T);
+      size_t lds = size_t(2 * P.N + P.N / 2) * sizeof(T);
       allow_big_lds(dwt_inv_whole<T, FILT>);
       hipLaunchKernelGGL((dwt_inv_whole<T, FILT>), dim3(1), dim3(kNTW), lds, st, a);
       WH_HIP_CHECK(hipGetLastError());
