@@ -178,25 +178,37 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
 }
 
 // ------------------------------------------------------------ inverse tile
+// A workgroup's work is a short chain of dependent phases, so its length,
+// not bandwidth, sets the pass's time; the kernel keeps that chain short:
+//   A  stage the y span and the top s_{j0+g} (every load issued first);
+//   B  copy each level's details out of the span into compact arrays (all
+//      LDS reads issued first, thresholded on the way);
+//   C  levels g..2, each reading s_l and d_l and writing s_{l-1} into the
+//      other of two buffers (one barrier a level instead of two);
+//   D  level 1 writes its outputs straight to memory.
 template <typename T, int FILT>
 __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
   constexpr int L = Filter<FILT>::L;
   constexpr int HP = L / 2 - 1;
   constexpr int PADS = 8;
-  // the y span the first pass needs: the tile and (L-2) 2^kGmax positions
-  // below it (the lowest detail any of its levels reads)
+  // the span every pass stages: the tile and (L-2) 2^kGmax positions below
+  // it (the lowest detail any of its levels reads)
   constexpr int HY = (L - 2) << kGmax;
-  using V = typename Vec2<T>::type;
+  constexpr int kSpan = kTile + HY;
   typedef T E2 __attribute__((ext_vector_type(2)));
-  // the pool holds the staged y span, then s of the current level: each
-  // level's outputs go through registers (at most kPairs pairs a thread)
-  // and back over its input after a barrier, so one buffer serves every
-  // level (four workgroups per CU)
-  constexpr int kPool = kTile + (HY > PADS ? HY : PADS);
-  constexpr int kPairs = (kTile / 2 + kNT - 1) / kNT;
+  // pool: the staged span; once the details are out, the two s buffers
+  // (A: s_1, s_3, ...; B: s_2, s_4, ...) share it
+  constexpr int kPool = kSpan > 2 ? kSpan : 2;
+  constexpr int kSA = kTile / 2 + 2 * L;  // s_1: kTile/2 + e_1 values
+  constexpr int kSB = kTile / 4 + 2 * L;  // s_2
+  static_assert(kSA + kSB <= kTile, "both s buffers fit the pool");
+  // dbuf: details of levels 1..g (level l at doff(l), own_l + e_l values
+  // and PADS spare), then the top s_g
+  constexpr int kDbuf = kTile + PADS * (kGmax + 1) + 2 * L;
   __shared__ __align__(16) T pool[kPool];
-  __shared__ T dbuf[kTile + PADS * kGmax];
-  T* bufS = pool;
+  __shared__ __align__(16) T dbuf[kDbuf];
+  T* bufA = pool;
+  T* bufB = pool + kSA;
 
   const int tid = threadIdx.x;
   const int g = a.g, j0 = a.j0;
@@ -214,13 +226,14 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
     for (int i = 1; i <= l; ++i) ee = ((ee / 2 + HP) + 1) & ~1;
     return ee;
   };
-  // Every pass stages the span [base - HY, base + kTile) of the level-j0
-  // sequence (y at stride 2^j0, positions mod N) in the pool: it holds
-  // every detail the tile's g levels read.  The first pass (j0 = 0) reads
-  // it with contiguous 16-byte loads; deeper passes with one load per
-  // value.  All these loads, and those of the top s_{j0+g}, are issued
-  // before any is used, so the staging costs one memory latency.
-  constexpr int kSpan = kTile + HY;
+  auto doff = [](int l) { return (kTile - (kTile >> (l - 1))) + PADS * (l - 1); };
+
+  // ---- A: the span [base - HY, base + kTile) of the level-j0 sequence (y
+  // at stride 2^j0, positions mod N) holds every detail the tile's g levels
+  // read.  The first pass (j0 = 0) reads it with contiguous 16-byte loads;
+  // deeper passes with one load per value.  Loads past the end are clamped
+  // duplicates: a guarded load would be merged with its guarded store and
+  // wait there.
   constexpr int kLdP = (kSpan / 2 + kNT - 1) / kNT;  // pair loads a thread (j0 = 0)
   constexpr int kLdS = (kSpan + kNT - 1) / kNT;      // single loads a thread (j0 > 0)
   constexpr int kTopMax = (kTile / 2 + L + kNT - 1) / kNT;
@@ -231,8 +244,9 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
   const int cnt_g = own_g + eg;
   const int64_t Bg = base >> g;
   const int64_t Ng = N >> g;
+  T* top = dbuf + doff(g + 1);
   // top s_{j0+g}: one load a thread unless the pass has fewer than kGmax
-  // levels (a uniform branch); loads past cnt_g are clamped duplicates
+  // levels (a uniform branch)
   T topv[kTopMax];
   auto load_top = [&] {
     topv[0] = src[((Bg - eg + (tid < cnt_g ? tid : cnt_g - 1)) & (Ng - 1)) << a.src_shift];
@@ -245,8 +259,8 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
       }
     }
   };
-  // Only the first tile's halo (or a sequence shorter than the span) wraps
-  // around; every other tile addresses its span from one base pointer.
+  // only the first tile's halo (or a sequence shorter than the span) wraps
+  // around; every other tile addresses its span from one base pointer
   const bool wrap = (base < HY) || (N < kSpan);
   if (j0 == 0) {
     E2 ld[kLdP];
@@ -255,13 +269,13 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
 #pragma unroll
       for (int k = 0; k < kLdP; ++k) {
         const int i = 2 * (tid + k * kNT);
-        ld[k] = *reinterpret_cast<const E2*>(ys + (i < span ? i : span - 2));  // clamped
+        ld[k] = *reinterpret_cast<const E2*>(ys + (i < span ? i : span - 2));
       }
     } else {
 #pragma unroll
       for (int k = 0; k < kLdP; ++k) {
         const int i = 2 * (tid + k * kNT);
-        const int ic = i < span ? i : span - 2;  // unconditional load, clamped
+        const int ic = i < span ? i : span - 2;
         ld[k] = *reinterpret_cast<const E2*>(y + ((sbase + ic) & (N - 1)));
       }
     }
@@ -278,13 +292,13 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
 #pragma unroll
       for (int k = 0; k < kLdS; ++k) {
         const int i = tid + k * kNT;
-        ld[k] = ys[uint64_t(i < span ? i : span - 1) << j0];  // clamped
+        ld[k] = ys[uint64_t(i < span ? i : span - 1) << j0];
       }
     } else {
 #pragma unroll
       for (int k = 0; k < kLdS; ++k) {
         const int i = tid + k * kNT;
-        const int ic = i < span ? i : span - 1;  // unconditional load, clamped
+        const int ic = i < span ? i : span - 1;
         ld[k] = y[((sbase + ic) & (N - 1)) << j0];
       }
     }
@@ -295,45 +309,68 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
       if (i < span) pool[i] = ld[k];
     }
   }
-  __syncthreads();
-  // details d_l over local [0, own_l + e_l): level-l index m = B_l - e_l + i
-  // at compact position (2m + 1) 2^{l-1}, pool slot (that - sbase) mod N =
-  // ((2i + 1) 2^{l-1} + HY - e_l 2^l) mod N (B_l 2^l = base)
-  const uint32_t pmask = N > int64_t(0xffffffff) ? 0xffffffffu : uint32_t(N - 1);
-  int doff = 0;
-  for (int l = 1; l <= g; ++l) {
-    const int own = kTile >> l;
-    const int el = evn(l);
-    const int cnt = own + el;
-    const uint32_t cl = uint32_t(HY - (el << l));
-    const T t = thr_level<T>(a.thr, j0 + l - 1);
-    for (int i = tid; i < cnt; i += kNT) {
-      T d = pool[((uint32_t(2 * i + 1) << (l - 1)) + cl) & pmask];
-      if (t >= T(0)) d = thresh_coef(d, t, a.thr.hardness);
-      dbuf[doff + i] = d;
-    }
-    doff += own + PADS;
-  }
-  __syncthreads();  // the pool is reused for s below
-  // top s_{j0+g}, same range as d_g
 #pragma unroll
   for (int k = 0; k < kTopMax; ++k) {
     const int i = tid + k * kNT;
-    if (i < cnt_g) bufS[i] = topv[k];
+    if (i < cnt_g) top[i] = topv[k];
   }
   __syncthreads();
-  // levels g..1: level l reads s_l, d_l (start B_l - e_l) and writes
-  // s_{l-1} over local [0, own_{l-1} + e_{l-1}) (start B_{l-1} - e_{l-1})
+
+  // ---- B: details d_l over local [0, own_l + e_l): level-l index
+  // m = B_l - e_l + i at compact position (2m + 1) 2^{l-1}, pool slot
+  // (that - sbase) mod N = ((2i + 1) 2^{l-1} + HY - e_l 2^l) mod N
+  // (B_l 2^l = base).  Every level's reads are issued before any write
+  // (for levels past g too: their clamped slots stay inside the span).
+  const uint32_t pmask = N > int64_t(0xffffffff) ? 0xffffffffu : uint32_t(N - 1);
+  constexpr int kIt1 = (kTile / 2 + L + kNT - 1) / kNT;
+  T ev[kGmax][kIt1];
+#pragma unroll
+  for (int l = 1; l <= kGmax; ++l) {
+    const int cnt = (kTile >> l) + evn(l);
+    const uint32_t cl = uint32_t(HY - (evn(l) << l));
+#pragma unroll
+    for (int k = 0; k < ((kTile >> l) + L + kNT - 1) / kNT; ++k) {
+      const int i = tid + k * kNT;
+      const int ic = i < cnt ? i : cnt - 1;
+      ev[l - 1][k] = pool[((uint32_t(2 * ic + 1) << (l - 1)) + cl) & pmask];
+    }
+  }
+#pragma unroll
+  for (int l = 1; l <= kGmax; ++l) {
+    if (l <= g) {
+      const int cnt = (kTile >> l) + evn(l);
+      const T t = thr_level<T>(a.thr, j0 + l - 1);
+      T* dl = dbuf + doff(l);
+#pragma unroll
+      for (int k = 0; k < ((kTile >> l) + L + kNT - 1) / kNT; ++k) {
+        const int i = tid + k * kNT;
+        if (i < cnt) {
+          T d = ev[l - 1][k];
+          if (t >= T(0)) d = thresh_coef(d, t, a.thr.hardness);
+          dl[i] = d;
+        }
+      }
+    }
+  }
+  __syncthreads();  // the pool now serves the s buffers
+
+  // ---- C, D: level l reads s_l (the top for l = g, else A for odd l, B
+  // for even) and d_l, both starting at B_l - e_l, and writes s_{l-1} over
+  // local [0, own_{l-1} + e_{l-1}) (start B_{l-1} - e_{l-1}); level 1
+  // writes x[base + 2p], x[base + 2p + 1] to memory
+  constexpr int kPairs = (kTile / 2 + kNT - 1) / kNT;
+  T* __restrict__ dst = static_cast<T*>(a.dst);
   for (int l = g; l >= 1; --l) {
-    T* cur = bufS;
-    int dof = 0;
-    for (int ll = 1; ll < l; ++ll) dof += (kTile >> ll) + PADS;
-    const T* dl = dbuf + dof;
+    const T* cur = (l == g) ? top : ((l & 1) ? bufA : bufB);
+    T* nxt = ((l - 1) & 1) ? bufA : bufB;
+    const T* dl = dbuf + doff(l);
     const int elm = evn(l - 1);
     const int npairs = ((kTile >> (l - 1)) + elm) / 2;
-    // pair p has global index m = B_l - e_{l-1}/2 + p, local e_l - e_{l-1}/2 + p
+    // pair p has level-l index m = B_l - e_{l-1}/2 + p, local e_l - e_{l-1}/2 + p
     const int off = evn(l) - elm / 2;
-    V res[kPairs];
+    // every pair is computed before any is written: the compiler cannot
+    // tell the buffers apart, so a write would hold back the next reads
+    E2 res[kPairs];
 #pragma unroll
     for (int k = 0; k < kPairs; ++k) {
       const int p = tid + k * kNT;
@@ -344,21 +381,26 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
           sv[q] = cur[off + p - q];
           dv[q] = dl[off + p - q];
         }
-        inv_pair<T, FILT>(sv, dv, res[k].x, res[k].y);
+        T e, o;
+        inv_pair<T, FILT>(sv, dv, e, o);
+        res[k] = E2{e, o};
       }
     }
-    __syncthreads();  // every s_l read before s_{l-1} overwrites it
+    if (l > 1) {
 #pragma unroll
-    for (int k = 0; k < kPairs; ++k) {
-      const int p = tid + k * kNT;
-      if (p < npairs) *reinterpret_cast<V*>(cur + 2 * p) = res[k];
+      for (int k = 0; k < kPairs; ++k) {
+        const int p = tid + k * kNT;
+        if (p < npairs) *reinterpret_cast<E2*>(nxt + 2 * p) = res[k];
+      }
+      __syncthreads();
+    } else {
+#pragma unroll
+      for (int k = 0; k < kPairs; ++k) {
+        const int p = tid + k * kNT;
+        if (p < npairs) *reinterpret_cast<E2*>(dst + base + 2 * p) = res[k];
+      }
     }
-    __syncthreads();
   }
-  // the pool holds s_{j0} for local [0, kTile)
-  T* __restrict__ dst = static_cast<T*>(a.dst);
-  for (int q = 2 * tid; q < kTile; q += 2 * kNT)
-    *reinterpret_cast<V*>(dst + base + q) = *reinterpret_cast<const V*>(bufS + q);
 }
 
 // ----------------------------------------------------------- whole kernels
