@@ -70,12 +70,12 @@ __device__ __forceinline__ int batch_next(int* counter, int* sh) {
 // write back and invalidate the L2 of the workgroup's XCD at every item.
 __device__ __forceinline__ void batch_phase() { __syncthreads(); }
 
-// MODWT item: fused forward of all J = G levels into scratch (D_j and S_J),
-// MAD of D_1, fused thresholding inverse into y.  Chunk width M = min(2 NT,
-// n); the warm-up covers the coarsest level's reach.
+// MODWT item, forward: the fused forward of all J = G levels into scratch
+// (D_j and S_J).  Chunk width M = min(2 NT, n); the warm-up covers the
+// coarsest level's reach.
 template <typename T, int FILT, int NT, int G>
-__device__ __forceinline__ void batch_modwt_item(const BatchArgs& b, const BatchItem& I, T* scr,
-                                                 unsigned char* smem) {
+__device__ __forceinline__ ModwtArgs batch_modwt_args(const BatchArgs& b, const BatchItem& I,
+                                                      T* scr) {
   constexpr int L = Filter<FILT>::L;
   const int64_t n = I.n;
   ModwtArgs a{};
@@ -89,10 +89,23 @@ __device__ __forceinline__ void batch_modwt_item(const BatchArgs& b, const Batch
   a.nchunks = int((n + a.warm) / a.M);
   a.in = static_cast<const T*>(b.x) + I.off;
   a.xmod = scr;
+  return a;
+}
+
+template <typename T, int FILT, int NT, int G>
+__device__ __forceinline__ void batch_modwt_fwd(const BatchArgs& b, const BatchItem& I, T* scr,
+                                                unsigned char* smem) {
+  const ModwtArgs a = batch_modwt_args<T, FILT, NT, G>(b, I, scr);
   modwt_fwd_body<T, FILT, 1, NT, G, 0, kStoreSmooth>(a, 0, 0, smem);
-  batch_phase();
-  const double t = block_mad<T, NT>(scr, n, 1, n, smem) * I.factor;
-  if (threadIdx.x == 0 && b.thresh_out) b.thresh_out[I.sig] = t;
+}
+
+// MODWT item, inverse: the fused inverse into y, thresholding every detail
+// level at t as it reads it
+template <typename T, int FILT, int NT, int G>
+__device__ __forceinline__ void batch_modwt_inv(const BatchArgs& b, const BatchItem& I, T* scr,
+                                                unsigned char* smem, double t) {
+  ModwtArgs a = batch_modwt_args<T, FILT, NT, G>(b, I, scr);
+  const int64_t n = I.n;
   a.stop = scr + 2 * n * (G - 1);
   a.out = static_cast<T*>(b.y) + I.off;
   a.th_on = 1;
@@ -102,7 +115,6 @@ __device__ __forceinline__ void batch_modwt_item(const BatchArgs& b, const Batch
   a.th_value = t;
   a.th_levels = nullptr;
   modwt_inv_lin_body<T, FILT, NT, G>(a, 0, smem);
-  batch_phase();
 }
 
 // LDS elements per ping-pong buffer of the DWT tail (64 KB in FP64, 32 KB
@@ -112,13 +124,13 @@ constexpr int batch_lds_elems() {
   return 4096;
 }
 
-// DWT item: levels run on compact sequences, through scratch while longer
-// than the LDS tile and in LDS afterwards; coefficients are kept in the
-// reference's interleaved layout in scratch C (n values, gA/gB n/2 each
-// behind it); MAD of the level-1 details; thresholding inverse into y.
+// DWT item, forward: levels run on compact sequences, through scratch while
+// longer than the LDS tile and in LDS afterwards; the coefficients end in
+// the reference's interleaved layout in scratch C (n values, gA/gB n/2 each
+// behind it).
 template <typename T, int FILT, int NT>
-__device__ __forceinline__ void batch_dwt_item(const BatchArgs& b, const BatchItem& I, T* C,
-                                               unsigned char* smem) {
+__device__ __forceinline__ void batch_dwt_fwd(const BatchArgs& b, const BatchItem& I, T* C,
+                                              unsigned char* smem) {
   constexpr int L = Filter<FILT>::L;
   constexpr int kBatchLds = batch_lds_elems<T>();
   constexpr int kU = 4;  // global-level pairs per thread in flight
@@ -196,12 +208,32 @@ __device__ __forceinline__ void batch_dwt_item(const BatchArgs& b, const BatchIt
     if (!all_global)
       for (int64_t m = tid; m < N; m += NT) C[m << J] = cur[m];
     batch_phase();
+  }
+}
 
-    // ---- universal threshold from the level-1 details
-    const double t = (J == 1) ? block_mad<T, NT>(C, 1, 1, n - 1, smem) * I.factor
-                              : block_mad<T, NT>(C, 1, 2, n / 2, smem) * I.factor;
-    if (tid == 0 && b.thresh_out) b.thresh_out[I.sig] = t;
+// DWT item, inverse: from the interleaved coefficients in scratch C, the
+// thresholding inverse into y (levels J.. in LDS while the output fits the
+// tile, the long levels through gA/gB)
+template <typename T, int FILT, int NT>
+__device__ __forceinline__ void batch_dwt_inv(const BatchArgs& b, const BatchItem& I, T* C,
+                                              unsigned char* smem, double t) {
+  constexpr int L = Filter<FILT>::L;
+  constexpr int kBatchLds = batch_lds_elems<T>();
+  constexpr int kU = 4;  // global-level pairs per thread in flight
+  T* lA = reinterpret_cast<T*>(smem);
+  T* lB = lA + kBatchLds;
+  const int tid = threadIdx.x;
+  {
+    const int64_t n = I.n;
+    const int J = I.J;
+    T* gA = C + n;          // n/2
+    T* gB = gA + n / 2;     // n/2
+    T* y = static_cast<T*>(b.y) + I.off;
     const T tt = T(t);
+    int64_t N;
+    T* cur;
+    T* nxt;
+    int j;
 
     // ---- inverse: levels J.. in LDS while the output fits the tile
     N = n >> J;  // length of s_J
@@ -285,21 +317,40 @@ __device__ __forceinline__ void batch_dwt_item(const BatchArgs& b, const BatchIt
 }
 
 template <typename T, int FILT, int NT>
-__device__ __forceinline__ void batch_item(const BatchArgs& b, const BatchItem& I, T* scr,
-                                           unsigned char* smem) {
+__device__ __forceinline__ void batch_fwd(const BatchArgs& b, const BatchItem& I, T* scr,
+                                          unsigned char* smem) {
   if (kind_ttype(I.kind) == kDWT) {
-    batch_dwt_item<T, FILT, NT>(b, I, scr, smem);
+    batch_dwt_fwd<T, FILT, NT>(b, I, scr, smem);
     return;
   }
   switch (I.J) {
-    case 1: batch_modwt_item<T, FILT, NT, 1>(b, I, scr, smem); break;
-    case 2: batch_modwt_item<T, FILT, NT, 2>(b, I, scr, smem); break;
-    case 3: batch_modwt_item<T, FILT, NT, 3>(b, I, scr, smem); break;
-    case 4: batch_modwt_item<T, FILT, NT, 4>(b, I, scr, smem); break;
-    case 5: batch_modwt_item<T, FILT, NT, 5>(b, I, scr, smem); break;
-    case 6: batch_modwt_item<T, FILT, NT, 6>(b, I, scr, smem); break;
-    case 7: batch_modwt_item<T, FILT, NT, 7>(b, I, scr, smem); break;
-    default: batch_modwt_item<T, FILT, NT, 8>(b, I, scr, smem); break;
+    case 1: batch_modwt_fwd<T, FILT, NT, 1>(b, I, scr, smem); break;
+    case 2: batch_modwt_fwd<T, FILT, NT, 2>(b, I, scr, smem); break;
+    case 3: batch_modwt_fwd<T, FILT, NT, 3>(b, I, scr, smem); break;
+    case 4: batch_modwt_fwd<T, FILT, NT, 4>(b, I, scr, smem); break;
+    case 5: batch_modwt_fwd<T, FILT, NT, 5>(b, I, scr, smem); break;
+    case 6: batch_modwt_fwd<T, FILT, NT, 6>(b, I, scr, smem); break;
+    case 7: batch_modwt_fwd<T, FILT, NT, 7>(b, I, scr, smem); break;
+    default: batch_modwt_fwd<T, FILT, NT, 8>(b, I, scr, smem); break;
+  }
+}
+
+template <typename T, int FILT, int NT>
+__device__ __forceinline__ void batch_inv(const BatchArgs& b, const BatchItem& I, T* scr,
+                                          unsigned char* smem, double t) {
+  if (kind_ttype(I.kind) == kDWT) {
+    batch_dwt_inv<T, FILT, NT>(b, I, scr, smem, t);
+    return;
+  }
+  switch (I.J) {
+    case 1: batch_modwt_inv<T, FILT, NT, 1>(b, I, scr, smem, t); break;
+    case 2: batch_modwt_inv<T, FILT, NT, 2>(b, I, scr, smem, t); break;
+    case 3: batch_modwt_inv<T, FILT, NT, 3>(b, I, scr, smem, t); break;
+    case 4: batch_modwt_inv<T, FILT, NT, 4>(b, I, scr, smem, t); break;
+    case 5: batch_modwt_inv<T, FILT, NT, 5>(b, I, scr, smem, t); break;
+    case 6: batch_modwt_inv<T, FILT, NT, 6>(b, I, scr, smem, t); break;
+    case 7: batch_modwt_inv<T, FILT, NT, 7>(b, I, scr, smem, t); break;
+    default: batch_modwt_inv<T, FILT, NT, 8>(b, I, scr, smem, t); break;
   }
 }
 
@@ -316,11 +367,29 @@ __global__ __launch_bounds__(NT) void batch_smooth_k(BatchArgs b) {
     if (it >= b.count) break;
     const BatchItem I = b.items[it];
     switch (kind_filter(I.kind)) {
-      case kHAAR: batch_item<T, kHAAR, NT>(b, I, scr, smem); break;
-      case kD4: batch_item<T, kD4, NT>(b, I, scr, smem); break;
-      case kC6: batch_item<T, kC6, NT>(b, I, scr, smem); break;
-      default: batch_item<T, kLA8, NT>(b, I, scr, smem); break;
+      case kHAAR: batch_fwd<T, kHAAR, NT>(b, I, scr, smem); break;
+      case kD4: batch_fwd<T, kD4, NT>(b, I, scr, smem); break;
+      case kC6: batch_fwd<T, kC6, NT>(b, I, scr, smem); break;
+      default: batch_fwd<T, kLA8, NT>(b, I, scr, smem); break;
     }
+    batch_phase();
+    // the universal threshold from the level-1 details: one call site for
+    // every kind of item, so the selection is inlined once (a called
+    // function would keep its saved registers in scratch memory)
+    const int64_t n = I.n;
+    const bool dwt = kind_ttype(I.kind) == kDWT;
+    const int64_t moff = dwt ? 1 : n;
+    const int64_t mstride = (dwt && I.J > 1) ? 2 : 1;
+    const int64_t mcount = dwt ? (I.J == 1 ? n - 1 : n / 2) : n;
+    const double t = block_mad<T, NT>(scr, moff, mstride, mcount, smem) * I.factor;
+    if (threadIdx.x == 0 && b.thresh_out) b.thresh_out[I.sig] = t;
+    switch (kind_filter(I.kind)) {
+      case kHAAR: batch_inv<T, kHAAR, NT>(b, I, scr, smem, t); break;
+      case kD4: batch_inv<T, kD4, NT>(b, I, scr, smem, t); break;
+      case kC6: batch_inv<T, kC6, NT>(b, I, scr, smem, t); break;
+      default: batch_inv<T, kLA8, NT>(b, I, scr, smem, t); break;
+    }
+    batch_phase();
   }
   if (threadIdx.x == 0) {
     __threadfence();
