@@ -39,6 +39,27 @@ constexpr int kWholeWork = 2048;  // longest sequence a "whole" kernel finishes
 
 __host__ __device__ constexpr int halo_fwd(int levels, int L) { return ((1 << levels) - 1) * (L - 2); }
 
+// Inverse halos: level l's arrays start e_l positions (of level l) below
+// the tile, e_l = h_l rounded up to even with h_l = e_{l-1}/2 + L/2 - 1 (the
+// inverse of level l+1 reads s_l and d_l that far back); a pair's two
+// outputs then never straddle an array start.  e_l <= L - 2.
+__host__ __device__ constexpr int inv_evn(int l, int L) {
+  int ee = 0;
+  for (int i = 1; i <= l; ++i) ee = ((ee / 2 + L / 2 - 1) + 1) & ~1;
+  return ee;
+}
+// The lowest position of the level-j0 sequence an inverse tile pass reads
+// below its tile: d_l from index B_l - e_l, at compact position
+// base - e_l 2^l + 2^{l-1}
+__host__ __device__ constexpr int inv_halo(int L) {
+  int h = 0;
+  for (int l = 1; l <= kGmax; ++l) {
+    const int v = (inv_evn(l, L) << l) - (1 << (l - 1));
+    h = v > h ? v : h;
+  }
+  return h;
+}
+
 struct Thr {
   int on, hardness, minlevel, maxlevel;
   double value;
@@ -178,37 +199,41 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
 }
 
 // ------------------------------------------------------------ inverse tile
-// A workgroup's work is a short chain of dependent phases, so its length,
-// not bandwidth, sets the pass's time; the kernel keeps that chain short:
-//   A  stage the y span and the top s_{j0+g} (every load issued first);
-//   B  copy each level's details out of the span into compact arrays (all
-//      LDS reads issued first, thresholded on the way);
-//   C  levels g..2, each reading s_l and d_l and writing s_{l-1} into the
-//      other of two buffers (one barrier a level instead of two);
-//   D  level 1 writes its outputs straight to memory.
+// A workgroup's work is a short chain of dependent phases, so its length
+// and the number of workgroups a CU holds, not bandwidth, set the pass's
+// time.  The kernel keeps the chain short and its LDS small:
+//   A  stage the y span (the level-j0 sequence around the tile) and the top
+//      s_{j0+g}, every load issued first;
+//   B  copy the details of levels 2..g out of the span into compact arrays
+//      (all LDS reads issued first, thresholded on the way);
+//   C  levels g..3 write s_{l-1} into two small buffers in turn, level 2
+//      writes s_1 into the span itself, at the positions of the coarser
+//      coefficients it no longer needs (one barrier a level);
+//   D  level 1 runs in place in the span: s_1 and d_1 sit side by side
+//      there, one 16-byte read a tap, and its outputs go straight to memory.
+// 32 KB of LDS in FP64 with LA8: five workgroups per CU.
 template <typename T, int FILT>
 __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
   constexpr int L = Filter<FILT>::L;
-  constexpr int HP = L / 2 - 1;
   constexpr int PADS = 8;
-  // the span every pass stages: the tile and (L-2) 2^kGmax positions below
-  // it (the lowest detail any of its levels reads)
-  constexpr int HY = (L - 2) << kGmax;
+  // the span every pass stages: the tile and the HY positions below it
+  // down to the lowest detail any of its levels reads (an even count)
+  constexpr int HY = inv_halo(L);
+  static_assert(HY % 2 == 0, "the span starts on a pair");
   constexpr int kSpan = kTile + HY;
   typedef T E2 __attribute__((ext_vector_type(2)));
-  // pool: the staged span; once the details are out, the two s buffers
-  // (A: s_1, s_3, ...; B: s_2, s_4, ...) share it
-  constexpr int kPool = kSpan > 2 ? kSpan : 2;
-  constexpr int kSA = kTile / 2 + 2 * L;  // s_1: kTile/2 + e_1 values
-  constexpr int kSB = kTile / 4 + 2 * L;  // s_2
-  static_assert(kSA + kSB <= kTile, "both s buffers fit the pool");
-  // dbuf: details of levels 1..g (level l at doff(l), own_l + e_l values
-  // and PADS spare), then the top s_g
-  constexpr int kDbuf = kTile + PADS * (kGmax + 1) + 2 * L;
-  __shared__ __align__(16) T pool[kPool];
-  __shared__ __align__(16) T dbuf[kDbuf];
-  T* bufA = pool;
-  T* bufB = pool + kSA;
+  // rest: d_2..d_kGmax (level l at doff(l), own_l + e_l values and PADS
+  // spare), then P0 (s_2, s_4) and P1 (s_3, s_5); the top s_g of a pass
+  // with g >= 2 goes where level g reads it (P0 or P1).  Sized to the
+  // byte: at 32,000 bytes (FP64, LA8) a CU holds five workgroups.
+  constexpr int kD = (kTile / 2 - (kTile >> kGmax)) + PADS * (kGmax - 1);
+  constexpr int kP0 = kTile / 4 + L - 2;  // s_2: own_2 + e_2
+  constexpr int kP1 = kTile / 8 + L - 2;  // s_3: own_3 + e_3
+  static_assert(kP0 % 2 == 0 && kP1 % 2 == 0 && kD % 2 == 0, "pair-aligned buffers");
+  __shared__ __align__(16) T pool[kSpan];
+  __shared__ __align__(16) T rest[kD + kP0 + kP1];
+  T* P0 = rest + kD;
+  T* P1 = P0 + kP0;
 
   const int tid = threadIdx.x;
   const int g = a.g, j0 = a.j0;
@@ -217,34 +242,35 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
   const T* __restrict__ src = static_cast<const T*>(a.src);
   const T* __restrict__ y = static_cast<const T*>(a.y);
 
-  // Halo below the tile at level l: the inverse of level l+1 needs s_l and
-  // d_l from h_l = e_{l-1}/2 + L/2 - 1 positions below the tile; every
-  // level-l buffer starts e_l = h_l rounded up to even positions below it,
-  // so a pair's two outputs never straddle a buffer start.  e_l <= L - 2.
-  auto evn = [](int l) {
-    int ee = 0;
-    for (int i = 1; i <= l; ++i) ee = ((ee / 2 + HP) + 1) & ~1;
-    return ee;
-  };
-  auto doff = [](int l) { return (kTile - (kTile >> (l - 1))) + PADS * (l - 1); };
+  auto evn = [](int l) { return inv_evn(l, L); };
+  auto doff = [](int l) { return (kTile / 2 - (kTile >> (l - 1))) + PADS * (l - 2); };
+  auto sbuf = [&](int j) -> T* { return (j & 1) ? P1 : P0; };  // s_j, j >= 2
 
-  // ---- A: the span [base - HY, base + kTile) of the level-j0 sequence (y
-  // at stride 2^j0, positions mod N) holds every detail the tile's g levels
-  // read.  The first pass (j0 = 0) reads it with contiguous 16-byte loads;
-  // deeper passes with one load per value.  Loads past the end are clamped
-  // duplicates: a guarded load would be merged with its guarded store and
-  // wait there.
+  // Local index i of level l's arrays (s_l, d_l) is level-l index
+  // m = B_l - e_l + i; s_l[m] sits at compact position m 2^l, d_l[m] at
+  // (2m + 1) 2^{l-1}, and the span slot of compact position c is
+  // (c - sbase) mod N, so d_l's slot is ((2i + 1) 2^{l-1} + HY - e_l 2^l)
+  // mod N (B_l 2^l = base) and s_1's is (2i + HY - 2 e_1) mod N.
+  const int64_t sbase = base - HY;  // compact position of pool[0]
+  const uint32_t pmask = N > int64_t(0xffffffff) ? 0xffffffffu : uint32_t(N - 1);
+  auto dslot = [&](int l, int i) -> uint32_t {
+    return ((uint32_t(2 * i + 1) << (l - 1)) + uint32_t(HY - (evn(l) << l))) & pmask;
+  };
+  const uint32_t c1 = uint32_t(HY - 2 * evn(1));  // s_1[i] at slot 2i + c1, d_1[i] next to it
+
+  // ---- A: the span [base - HY, base + kTile).  The first pass (j0 = 0)
+  // reads it with contiguous 16-byte loads; deeper passes with one load per
+  // value.  Loads past the end are clamped duplicates: a guarded load would
+  // be merged with its guarded store and wait there.
   constexpr int kLdP = (kSpan / 2 + kNT - 1) / kNT;  // pair loads a thread (j0 = 0)
   constexpr int kLdS = (kSpan + kNT - 1) / kNT;      // single loads a thread (j0 > 0)
   constexpr int kTopMax = (kTile / 2 + L + kNT - 1) / kNT;
   const int span = int(std::min<int64_t>(kSpan, N));  // pool positions used
-  const int64_t sbase = base - HY;                    // compact position of pool[0]
   const int own_g = kTile >> g;
   const int eg = evn(g);
   const int cnt_g = own_g + eg;
   const int64_t Bg = base >> g;
   const int64_t Ng = N >> g;
-  T* top = dbuf + doff(g + 1);
   // top s_{j0+g}: one load a thread unless the pass has fewer than kGmax
   // levels (a uniform branch)
   T topv[kTopMax];
@@ -309,67 +335,71 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
       if (i < span) pool[i] = ld[k];
     }
   }
+  if (g >= 2) {
+    T* top = sbuf(g);
 #pragma unroll
-  for (int k = 0; k < kTopMax; ++k) {
-    const int i = tid + k * kNT;
-    if (i < cnt_g) top[i] = topv[k];
-  }
-  __syncthreads();
-
-  // ---- B: details d_l over local [0, own_l + e_l): level-l index
-  // m = B_l - e_l + i at compact position (2m + 1) 2^{l-1}, pool slot
-  // (that - sbase) mod N = ((2i + 1) 2^{l-1} + HY - e_l 2^l) mod N
-  // (B_l 2^l = base).  Every level's reads are issued before any write
-  // (for levels past g too: their clamped slots stay inside the span).
-  const uint32_t pmask = N > int64_t(0xffffffff) ? 0xffffffffu : uint32_t(N - 1);
-  constexpr int kIt1 = (kTile / 2 + L + kNT - 1) / kNT;
-  T ev[kGmax][kIt1];
-#pragma unroll
-  for (int l = 1; l <= kGmax; ++l) {
-    const int cnt = (kTile >> l) + evn(l);
-    const uint32_t cl = uint32_t(HY - (evn(l) << l));
-#pragma unroll
-    for (int k = 0; k < ((kTile >> l) + L + kNT - 1) / kNT; ++k) {
+    for (int k = 0; k < kTopMax; ++k) {
       const int i = tid + k * kNT;
-      const int ic = i < cnt ? i : cnt - 1;
-      ev[l - 1][k] = pool[((uint32_t(2 * ic + 1) << (l - 1)) + cl) & pmask];
+      if (i < cnt_g) top[i] = topv[k];
     }
   }
+  __syncthreads();
+  if (g == 1) {
+    // a one-level pass: its top s_1 goes where level 1 reads it, over the
+    // span's coarser positions (staged above, hence the extra barrier)
 #pragma unroll
-  for (int l = 1; l <= kGmax; ++l) {
-    if (l <= g) {
+    for (int k = 0; k < kTopMax; ++k) {
+      const int i = tid + k * kNT;
+      if (i < cnt_g) pool[(uint32_t(2 * i) + c1) & pmask] = topv[k];
+    }
+    __syncthreads();
+  }
+
+  // ---- B: d_2..d_g out of the span, every read issued first (for levels
+  // past g too: their clamped slots stay inside the span)
+  if (g >= 2) {
+    constexpr int kIt2 = (kTile / 4 + L + kNT - 1) / kNT;
+    T ev[kGmax][kIt2];
+#pragma unroll
+    for (int l = 2; l <= kGmax; ++l) {
       const int cnt = (kTile >> l) + evn(l);
-      const T t = thr_level<T>(a.thr, j0 + l - 1);
-      T* dl = dbuf + doff(l);
 #pragma unroll
       for (int k = 0; k < ((kTile >> l) + L + kNT - 1) / kNT; ++k) {
         const int i = tid + k * kNT;
-        if (i < cnt) {
-          T d = ev[l - 1][k];
-          if (t >= T(0)) d = thresh_coef(d, t, a.thr.hardness);
-          dl[i] = d;
+        ev[l - 1][k] = pool[dslot(l, i < cnt ? i : cnt - 1)];
+      }
+    }
+#pragma unroll
+    for (int l = 2; l <= kGmax; ++l) {
+      if (l <= g) {
+        const int cnt = (kTile >> l) + evn(l);
+        const T t = thr_level<T>(a.thr, j0 + l - 1);
+        T* dl = rest + doff(l);
+#pragma unroll
+        for (int k = 0; k < ((kTile >> l) + L + kNT - 1) / kNT; ++k) {
+          const int i = tid + k * kNT;
+          if (i < cnt) {
+            T d = ev[l - 1][k];
+            if (t >= T(0)) d = thresh_coef(d, t, a.thr.hardness);
+            dl[i] = d;
+          }
         }
       }
     }
+    __syncthreads();
   }
-  __syncthreads();  // the pool now serves the s buffers
 
-  // ---- C, D: level l reads s_l (the top for l = g, else A for odd l, B
-  // for even) and d_l, both starting at B_l - e_l, and writes s_{l-1} over
-  // local [0, own_{l-1} + e_{l-1}) (start B_{l-1} - e_{l-1}); level 1
-  // writes x[base + 2p], x[base + 2p + 1] to memory
+  // ---- C: levels g..2.  Pair p of level l has local index
+  // e_l - e_{l-1}/2 + p in s_l and d_l, and outputs local 2p, 2p + 1 of
+  // s_{l-1}.  Every pair is computed before any is written: the compiler
+  // cannot tell the buffers apart, so a write would hold back later reads.
   constexpr int kPairs = (kTile / 2 + kNT - 1) / kNT;
-  T* __restrict__ dst = static_cast<T*>(a.dst);
-  for (int l = g; l >= 1; --l) {
-    const T* cur = (l == g) ? top : ((l & 1) ? bufA : bufB);
-    T* nxt = ((l - 1) & 1) ? bufA : bufB;
-    const T* dl = dbuf + doff(l);
+  for (int l = g; l >= 2; --l) {
+    const T* cur = sbuf(l);
+    const T* dl = rest + doff(l);
     const int elm = evn(l - 1);
     const int npairs = ((kTile >> (l - 1)) + elm) / 2;
-    // pair p has level-l index m = B_l - e_{l-1}/2 + p, local e_l - e_{l-1}/2 + p
     const int off = evn(l) - elm / 2;
-    // every pair is computed before any is written: the compiler cannot
-    // tell the buffers apart, so a write would hold back the next reads
     E2 res[kPairs];
 #pragma unroll
     for (int k = 0; k < kPairs; ++k) {
@@ -386,19 +416,53 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
         res[k] = E2{e, o};
       }
     }
-    if (l > 1) {
+    if (l > 2) {
+      T* nxt = sbuf(l - 1);
 #pragma unroll
       for (int k = 0; k < kPairs; ++k) {
         const int p = tid + k * kNT;
         if (p < npairs) *reinterpret_cast<E2*>(nxt + 2 * p) = res[k];
       }
-      __syncthreads();
     } else {
 #pragma unroll
       for (int k = 0; k < kPairs; ++k) {
         const int p = tid + k * kNT;
-        if (p < npairs) *reinterpret_cast<E2*>(dst + base + 2 * p) = res[k];
+        if (p < npairs) {
+          pool[(uint32_t(4 * p) + c1) & pmask] = res[k].x;
+          pool[(uint32_t(4 * p + 2) + c1) & pmask] = res[k].y;
+        }
       }
+    }
+    __syncthreads();
+  }
+
+  // ---- D: level 1 in place in the span; outputs x[base + 2p], x[base + 2p + 1]
+  {
+    const int npairs = kTile / 2;  // e_0 = 0
+    const int off = evn(1);
+    const T t = thr_level<T>(a.thr, j0);
+    T* __restrict__ dst = static_cast<T*>(a.dst);
+    E2 res[kPairs];
+#pragma unroll
+    for (int k = 0; k < kPairs; ++k) {
+      const int p = tid + k * kNT;
+      if (p < npairs) {
+        T sv[L / 2], dv[L / 2];
+#pragma unroll
+        for (int q = 0; q < L / 2; ++q) {
+          const E2 v = *reinterpret_cast<const E2*>(pool + ((uint32_t(2 * (off + p - q)) + c1) & pmask));
+          sv[q] = v.x;
+          dv[q] = (t >= T(0)) ? thresh_coef(v.y, t, a.thr.hardness) : v.y;
+        }
+        T e, o;
+        inv_pair<T, FILT>(sv, dv, e, o);
+        res[k] = E2{e, o};
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kPairs; ++k) {
+      const int p = tid + k * kNT;
+      if (p < npairs) *reinterpret_cast<E2*>(dst + base + 2 * p) = res[k];
     }
   }
 }
