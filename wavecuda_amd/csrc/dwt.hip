@@ -93,12 +93,18 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
   constexpr int SZA = kTile + HMAX;
   // outputs per thread at level 1 (the widest level); each level's s is
   // computed into registers and written back over its input after a
-  // barrier, so one buffer serves every level (four workgroups per CU)
+  // barrier, so one buffer serves every level.  The level-1 details stay in
+  // the registers of the thread that writes them out (thread t computes
+  // d_1[t + k kNT] and writes y positions 2 (t + k kNT) and the next), so
+  // LDS holds only the deeper levels' details: 26 KB in FP64, six
+  // workgroups per CU.
   constexpr int kPer = (kTile / 2 + halo_fwd(kGmax - 1, L) + kNT - 1) / kNT;
+  constexpr int kPer1 = kTile / 2 / kNT;  // level-1 details a thread keeps
+  static_assert(kPer1 * kNT == kTile / 2, "whole level-1 pairs per thread");
   using V = typename Vec2<T>::type;
   typedef T E2 __attribute__((ext_vector_type(2)));
   __shared__ __align__(16) T bufA[SZA + 2];
-  __shared__ __align__(16) T dbuf[kTile];
+  __shared__ __align__(16) T dbuf[kTile / 2];  // level l >= 2 at kTile/2 - (kTile >> (l-1))
 
   const int tid = threadIdx.x;
   const int g = a.g;
@@ -136,10 +142,11 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
   __syncthreads();
 
   T* cur = bufA;
+  T d1[kPer1];
   for (int l = 1; l <= g; ++l) {
     const int own = kTile >> l;
     const int Kl = own + halo_fwd(g - l, L);
-    const int doff = kTile - (kTile >> (l - 1));
+    const int doff = kTile / 2 - (kTile >> (l - 1));  // levels >= 2
     T sreg[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
@@ -156,7 +163,11 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
         T sv, d;
         fwd_pair<T, FILT>(v, sv, d);
         sreg[k] = sv;
-        if (i < own) dbuf[doff + i] = d;
+        if (l == 1) {
+          if (k < kPer1) d1[k < kPer1 ? k : 0] = d;  // i < own exactly when k < kPer1
+        } else if (i < own) {
+          dbuf[doff + i] = d;
+        }
       }
     }
     __syncthreads();  // every tap of this level read before s_l overwrites it
@@ -167,28 +178,24 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
     }
     __syncthreads();
   }
-  // cur holds s_{j0+g}; own part is cur[0 .. kTile >> g)
+  // cur holds s_{j0+g}; own part is cur[0 .. kTile >> g).  Thread t writes
+  // the pairs of positions q = 2 (t + k kNT), q + 1: q holds a coarser
+  // coefficient (a deeper detail, or s when its ctz is at least g), q + 1
+  // the level-1 detail the thread kept.
   const int j0 = a.j0;
-  if (j0 == 0 && a.write_coarse) {
-    // first pass: every position of the tile, as 16-byte pairs (q even:
-    // a coarser coefficient; q + 1: a level-1 detail)
-    for (int qq = tid; qq < kTile / 2; qq += kNT) {
-      const int q = 2 * qq;
-      const int z = (q == 0) ? 31 : __builtin_ctz(unsigned(q));
-      const T v0 = (z >= g) ? cur[q >> g] : dbuf[(kTile - (kTile >> z)) + (q >> (z + 1))];
-      *reinterpret_cast<V*>(y + base + q) = V{v0, dbuf[qq]};
-    }
-  } else {
-    for (int q = tid; q < kTile; q += kNT) {
-      int z = (q == 0) ? 31 : __builtin_ctz(unsigned(q));
-      T v;
-      if (z >= g) {
-        if (!(a.final_pass || a.write_coarse)) continue;
-        v = cur[q >> g];
-      } else {
-        v = dbuf[(kTile - (kTile >> z)) + (q >> (z + 1))];
-      }
-      y[(base + q) << j0] = v;
+  const bool coarse = a.final_pass || a.write_coarse;
+#pragma unroll
+  for (int k = 0; k < kPer1; ++k) {
+    const int qq = tid + k * kNT;
+    const int q = 2 * qq;
+    const int z = (q == 0) ? 31 : __builtin_ctz(unsigned(q));
+    const T v0 = (z >= g) ? cur[q >> g] : dbuf[(kTile / 2 - (kTile >> z)) + (q >> (z + 1))];
+    if (j0 == 0 && a.write_coarse) {
+      // first pass: every position of the tile, as 16-byte pairs
+      *reinterpret_cast<V*>(y + base + q) = V{v0, d1[k]};
+    } else {
+      if (z < g || coarse) y[(base + q) << j0] = v0;
+      y[(base + q + 1) << j0] = d1[k];
     }
   }
   if (!a.final_pass) {
