@@ -211,14 +211,16 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
 // time.  The kernel keeps the chain short and its LDS small:
 //   A  stage the y span (the level-j0 sequence around the tile) and the top
 //      s_{j0+g}, every load issued first;
-//   B  copy the details of levels 2..g out of the span into compact arrays
+//   B  copy the details of levels 3..g out of the span into compact arrays
 //      (all LDS reads issued first, thresholded on the way);
-//   C  levels g..3 write s_{l-1} into two small buffers in turn, level 2
-//      writes s_1 into the span itself, at the positions of the coarser
+//   C  levels g..4 write s_{l-1} into two small buffers in turn, level 3
+//      writes s_2 into the span itself, at the positions of the coarser
 //      coefficients it no longer needs (one barrier a level);
-//   D  level 1 runs in place in the span: s_1 and d_1 sit side by side
-//      there, one 16-byte read a tap, and its outputs go straight to memory.
-// 32 KB of LDS in FP64 with LA8: five workgroups per CU.
+//   D  level 2 runs in place in the span (s_2 and d_2 at positions 4i and
+//      4i + 2 of the level-j0 sequence, s_1 written back over them after a
+//      barrier), level 1 too: s_1 and d_1 sit side by side there, one
+//      16-byte read a tap, and its outputs go straight to memory.
+// 25 KB of LDS in FP64 with LA8: six workgroups per CU.
 template <typename T, int FILT>
 __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
   constexpr int L = Filter<FILT>::L;
@@ -229,13 +231,12 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
   static_assert(HY % 2 == 0, "the span starts on a pair");
   constexpr int kSpan = kTile + HY;
   typedef T E2 __attribute__((ext_vector_type(2)));
-  // rest: d_2..d_kGmax (level l at doff(l), own_l + e_l values and PADS
-  // spare), then P0 (s_2, s_4) and P1 (s_3, s_5); the top s_g of a pass
-  // with g >= 2 goes where level g reads it (P0 or P1).  Sized to the
-  // byte: at 32,000 bytes (FP64, LA8) a CU holds five workgroups.
-  constexpr int kD = (kTile / 2 - (kTile >> kGmax)) + PADS * (kGmax - 1);
-  constexpr int kP0 = kTile / 4 + L - 2;  // s_2: own_2 + e_2
-  constexpr int kP1 = kTile / 8 + L - 2;  // s_3: own_3 + e_3
+  // rest: d_3..d_kGmax (level l at doff(l), own_l + e_l values and PADS
+  // spare), then P0 (s_4) and P1 (s_3, s_5); the top s_g of a pass with
+  // g >= 3 goes where level g reads it (P0 or P1)
+  constexpr int kD = (kTile / 4 - (kTile >> kGmax)) + PADS * (kGmax - 2);
+  constexpr int kP0 = kTile / 16 + L - 2;  // s_4: own_4 + e_4
+  constexpr int kP1 = kTile / 8 + L - 2;   // s_3: own_3 + e_3
   static_assert(kP0 % 2 == 0 && kP1 % 2 == 0 && kD % 2 == 0, "pair-aligned buffers");
   __shared__ __align__(16) T pool[kSpan];
   __shared__ __align__(16) T rest[kD + kP0 + kP1];
@@ -250,20 +251,23 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
   const T* __restrict__ y = static_cast<const T*>(a.y);
 
   auto evn = [](int l) { return inv_evn(l, L); };
-  auto doff = [](int l) { return (kTile / 2 - (kTile >> (l - 1))) + PADS * (l - 2); };
-  auto sbuf = [&](int j) -> T* { return (j & 1) ? P1 : P0; };  // s_j, j >= 2
+  auto doff = [](int l) { return (kTile / 4 - (kTile >> (l - 1))) + PADS * (l - 3); };
+  auto sbuf = [&](int j) -> T* { return (j & 1) ? P1 : P0; };  // s_j, j >= 3
 
   // Local index i of level l's arrays (s_l, d_l) is level-l index
   // m = B_l - e_l + i; s_l[m] sits at compact position m 2^l, d_l[m] at
   // (2m + 1) 2^{l-1}, and the span slot of compact position c is
   // (c - sbase) mod N, so d_l's slot is ((2i + 1) 2^{l-1} + HY - e_l 2^l)
-  // mod N (B_l 2^l = base) and s_1's is (2i + HY - 2 e_1) mod N.
+  // mod N (B_l 2^l = base), s_1's is (2i + HY - 2 e_1) mod N and s_2's
+  // (4i + HY - 4 e_2) mod N.
   const int64_t sbase = base - HY;  // compact position of pool[0]
   const uint32_t pmask = N > int64_t(0xffffffff) ? 0xffffffffu : uint32_t(N - 1);
   auto dslot = [&](int l, int i) -> uint32_t {
     return ((uint32_t(2 * i + 1) << (l - 1)) + uint32_t(HY - (evn(l) << l))) & pmask;
   };
   const uint32_t c1 = uint32_t(HY - 2 * evn(1));  // s_1[i] at slot 2i + c1, d_1[i] next to it
+  const uint32_t c2 = uint32_t(HY - 4 * evn(2));  // s_2[i] at slot 4i + c2, d_2[i] 2 further
+  static_assert(HY >= 4 * inv_evn(2, L) && HY >= 2 * inv_evn(1, L), "s_1, s_2 inside the span");
 
   // ---- A: the span [base - HY, base + kTile).  The first pass (j0 = 0)
   // reads it with contiguous 16-byte loads; deeper passes with one load per
@@ -342,7 +346,7 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
       if (i < span) pool[i] = ld[k];
     }
   }
-  if (g >= 2) {
+  if (g >= 3) {
     T* top = sbuf(g);
 #pragma unroll
     for (int k = 0; k < kTopMax; ++k) {
@@ -351,24 +355,25 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
     }
   }
   __syncthreads();
-  if (g == 1) {
-    // a one-level pass: its top s_1 goes where level 1 reads it, over the
-    // span's coarser positions (staged above, hence the extra barrier)
+  if (g <= 2) {
+    // a one- or two-level pass: its top goes where level g reads it, over
+    // the span's coarser positions (staged above, hence the extra barrier)
+    const uint32_t cg = g == 1 ? c1 : c2;
 #pragma unroll
     for (int k = 0; k < kTopMax; ++k) {
       const int i = tid + k * kNT;
-      if (i < cnt_g) pool[(uint32_t(2 * i) + c1) & pmask] = topv[k];
+      if (i < cnt_g) pool[((uint32_t(i) << g) + cg) & pmask] = topv[k];
     }
     __syncthreads();
   }
 
-  // ---- B: d_2..d_g out of the span, every read issued first (for levels
+  // ---- B: d_3..d_g out of the span, every read issued first (for levels
   // past g too: their clamped slots stay inside the span)
-  if (g >= 2) {
-    constexpr int kIt2 = (kTile / 4 + L + kNT - 1) / kNT;
-    T ev[kGmax][kIt2];
+  if (g >= 3) {
+    constexpr int kIt3 = (kTile / 8 + L + kNT - 1) / kNT;
+    T ev[kGmax][kIt3];
 #pragma unroll
-    for (int l = 2; l <= kGmax; ++l) {
+    for (int l = 3; l <= kGmax; ++l) {
       const int cnt = (kTile >> l) + evn(l);
 #pragma unroll
       for (int k = 0; k < ((kTile >> l) + L + kNT - 1) / kNT; ++k) {
@@ -377,7 +382,7 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
       }
     }
 #pragma unroll
-    for (int l = 2; l <= kGmax; ++l) {
+    for (int l = 3; l <= kGmax; ++l) {
       if (l <= g) {
         const int cnt = (kTile >> l) + evn(l);
         const T t = thr_level<T>(a.thr, j0 + l - 1);
@@ -396,12 +401,12 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
     __syncthreads();
   }
 
-  // ---- C: levels g..2.  Pair p of level l has local index
+  // ---- C: levels g..3.  Pair p of level l has local index
   // e_l - e_{l-1}/2 + p in s_l and d_l, and outputs local 2p, 2p + 1 of
   // s_{l-1}.  Every pair is computed before any is written: the compiler
   // cannot tell the buffers apart, so a write would hold back later reads.
   constexpr int kPairs = (kTile / 2 + kNT - 1) / kNT;
-  for (int l = g; l >= 2; --l) {
+  for (int l = g; l >= 3; --l) {
     const T* cur = sbuf(l);
     const T* dl = rest + doff(l);
     const int elm = evn(l - 1);
@@ -423,7 +428,7 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
         res[k] = E2{e, o};
       }
     }
-    if (l > 2) {
+    if (l > 3) {
       T* nxt = sbuf(l - 1);
 #pragma unroll
       for (int k = 0; k < kPairs; ++k) {
@@ -435,9 +440,45 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
       for (int k = 0; k < kPairs; ++k) {
         const int p = tid + k * kNT;
         if (p < npairs) {
-          pool[(uint32_t(4 * p) + c1) & pmask] = res[k].x;
-          pool[(uint32_t(4 * p + 2) + c1) & pmask] = res[k].y;
+          pool[(uint32_t(8 * p) + c2) & pmask] = res[k].x;
+          pool[(uint32_t(8 * p + 4) + c2) & pmask] = res[k].y;
         }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- level 2 in place: read every tap, barrier, write s_1 over them
+  if (g >= 2) {
+    const int elm = evn(1);
+    const int npairs = ((kTile >> 1) + elm) / 2;
+    const int off = evn(2) - elm / 2;
+    const T t = thr_level<T>(a.thr, j0 + 1);
+    E2 res[kPairs];
+#pragma unroll
+    for (int k = 0; k < kPairs; ++k) {
+      const int p = tid + k * kNT;
+      if (p < npairs) {
+        T sv[L / 2], dv[L / 2];
+#pragma unroll
+        for (int q = 0; q < L / 2; ++q) {
+          const uint32_t sl = uint32_t(4 * (off + p - q)) + c2;
+          sv[q] = pool[sl & pmask];
+          const T d = pool[(sl + 2) & pmask];
+          dv[q] = (t >= T(0)) ? thresh_coef(d, t, a.thr.hardness) : d;
+        }
+        T e, o;
+        inv_pair<T, FILT>(sv, dv, e, o);
+        res[k] = E2{e, o};
+      }
+    }
+    __syncthreads();  // every tap read before s_1 overwrites them
+#pragma unroll
+    for (int k = 0; k < kPairs; ++k) {
+      const int p = tid + k * kNT;
+      if (p < npairs) {
+        pool[(uint32_t(4 * p) + c1) & pmask] = res[k].x;
+        pool[(uint32_t(4 * p + 2) + c1) & pmask] = res[k].y;
       }
     }
     __syncthreads();
