@@ -124,10 +124,20 @@ constexpr int batch_lds_elems() {
   return 4096;
 }
 
+// DWT item coefficients in scratch C, level by level and each level
+// contiguous: d_j at C[n - n/2^{j-1} ..) (n/2^j values), then s_J; for
+// J = 1, s_1[0] goes last, so that C[0 .. n-1) holds d_1 and s_1[1..]: the
+// values of the reference's interleaved layout at positions 1..n-1, which
+// its one-level universal threshold reads (src/thresh.cpp:170-201).
+__device__ __forceinline__ int64_t batch_dwt_doff(int64_t n, int j) { return n - (n >> (j - 1)); }
+__device__ __forceinline__ int64_t batch_dwt_sidx(int64_t n, int J, int64_t m) {
+  if (J == 1) return m == 0 ? n - 1 : n / 2 + m - 1;
+  return n - (n >> J) + m;
+}
+
 // DWT item, forward: levels run on compact sequences, through scratch while
 // longer than the LDS tile and in LDS afterwards; the coefficients end in
-// the reference's interleaved layout in scratch C (n values, gA/gB n/2 each
-// behind it).
+// scratch C as above (n values, gA/gB n/2 each behind it).
 template <typename T, int FILT, int NT>
 __device__ __forceinline__ void batch_dwt_fwd(const BatchArgs& b, const BatchItem& I, T* C,
                                               unsigned char* smem) {
@@ -168,7 +178,7 @@ __device__ __forceinline__ void batch_dwt_fwd(const BatchArgs& b, const BatchIte
             T s, d;
             fwd_pair<T, FILT>(v[u], s, d);
             dst[m] = s;
-            C[(2 * m + 1) << (j - 1)] = d;
+            C[batch_dwt_doff(n, j) + m] = d;
           }
         }
       }
@@ -183,7 +193,7 @@ __device__ __forceinline__ void batch_dwt_fwd(const BatchArgs& b, const BatchIte
     T* nxt = lB;
     const bool all_global = (j > J);
     if (all_global) {
-      for (int64_t m = tid; m < N; m += NT) C[m << J] = src[m];
+      for (int64_t m = tid; m < N; m += NT) C[batch_dwt_sidx(n, J, m)] = src[m];
     } else {
       for (int64_t i = tid; i < N; i += NT) lA[i] = src[i];
     }
@@ -197,7 +207,7 @@ __device__ __forceinline__ void batch_dwt_fwd(const BatchArgs& b, const BatchIte
         T s, d;
         fwd_pair<T, FILT>(v, s, d);
         nxt[m] = s;
-        C[int64_t(2 * m + 1) << (j - 1)] = d;
+        C[batch_dwt_doff(n, j) + m] = d;
       }
       __syncthreads();
       T* t = cur;
@@ -206,7 +216,7 @@ __device__ __forceinline__ void batch_dwt_fwd(const BatchArgs& b, const BatchIte
       N = P;
     }
     if (!all_global)
-      for (int64_t m = tid; m < N; m += NT) C[m << J] = cur[m];
+      for (int64_t m = tid; m < N; m += NT) C[batch_dwt_sidx(n, J, m)] = cur[m];
     batch_phase();
   }
 }
@@ -239,9 +249,9 @@ __device__ __forceinline__ void batch_dwt_inv(const BatchArgs& b, const BatchIte
     N = n >> J;  // length of s_J
     const bool lds_tail = (2 * N <= kBatchLds);
     if (lds_tail) {
-      for (int64_t m = tid; m < N; m += NT) lA[m] = C[m << J];
+      for (int64_t m = tid; m < N; m += NT) lA[m] = C[batch_dwt_sidx(n, J, m)];
     } else {
-      for (int64_t m = tid; m < N; m += NT) gA[m] = C[m << J];
+      for (int64_t m = tid; m < N; m += NT) gA[m] = C[batch_dwt_sidx(n, J, m)];
     }
     batch_phase();
     cur = lA;
@@ -254,7 +264,7 @@ __device__ __forceinline__ void batch_dwt_inv(const BatchArgs& b, const BatchIte
         for (int q = 0; q < L / 2; ++q) {
           const int mm = (m - q) & int(N - 1);
           sv[q] = cur[mm];
-          dv[q] = thresh_coef(C[int64_t(2 * mm + 1) << (j - 1)], tt, b.hardness);
+          dv[q] = thresh_coef(C[batch_dwt_doff(n, j) + mm], tt, b.hardness);
         }
         T e, o;
         inv_pair<T, FILT>(sv, dv, e, o);
@@ -290,7 +300,7 @@ __device__ __forceinline__ void batch_dwt_inv(const BatchArgs& b, const BatchIte
             for (int q = 0; q < L / 2; ++q) {
               const int64_t mm = (m - q) & (N - 1);
               sv[u][q] = (m < N) ? s[mm] : T(0);
-              dv[u][q] = (m < N) ? C[(2 * mm + 1) << (j - 1)] : T(0);
+              dv[u][q] = (m < N) ? C[batch_dwt_doff(n, j) + mm] : T(0);
             }
           }
 #pragma unroll
@@ -378,8 +388,8 @@ __global__ __launch_bounds__(NT) void batch_smooth_k(BatchArgs b) {
     // function would keep its saved registers in scratch memory)
     const int64_t n = I.n;
     const bool dwt = kind_ttype(I.kind) == kDWT;
-    const int64_t moff = dwt ? 1 : n;
-    const int64_t mstride = (dwt && I.J > 1) ? 2 : 1;
+    const int64_t moff = dwt ? 0 : n;
+    const int64_t mstride = 1;
     const int64_t mcount = dwt ? (I.J == 1 ? n - 1 : n / 2) : n;
     const double t = block_mad<T, NT>(scr, moff, mstride, mcount, smem) * I.factor;
     if (threadIdx.x == 0 && b.thresh_out) b.thresh_out[I.sig] = t;
