@@ -80,8 +80,13 @@ def test_modwt_parity_f32(filt, n, nlev):
     assert O.relerr(xr, x) <= TOL32
 
 
-DWT_CASES = [(2, 0), (4, 0), (8, 0), (16, 0), (64, 0), (1024, 0), (1024, 3), (8192, 0),
-             (16384, 0), (16384, 7), (1 << 16, 0), (1 << 16, 4), (1 << 18, 0), (1 << 20, 12)]
+# (4096, 1..3): one tile pass of 1..3 levels over the signal itself;
+# (2^17, 6..8): a five-level tile pass, then a last tile pass of 1..3
+# levels (the inverse's in-place top s for one and two levels)
+DWT_CASES = [(2, 0), (4, 0), (8, 0), (16, 0), (64, 0), (1024, 0), (1024, 3), (4096, 1),
+             (4096, 2), (4096, 3), (8192, 0), (16384, 0), (16384, 7), (1 << 16, 0),
+             (1 << 16, 4), (1 << 17, 6), (1 << 17, 7), (1 << 17, 8), (1 << 18, 0),
+             (1 << 20, 12)]
 
 
 @pytest.mark.parametrize("filt", FILTERS)
@@ -103,7 +108,34 @@ def test_dwt_parity_f64(filt, n, nlev):
 
 
 @pytest.mark.parametrize("filt", FILTERS)
-@pytest.mark.parametrize("n,nlev", [(1024, 0), (16384, 0), (1 << 18, 0)])
+@pytest.mark.parametrize("n,nlev", [(4096, 1), (4096, 3), (1 << 16, 4), (1 << 17, 6),
+                                    (1 << 17, 7), (1 << 18, 0)])
+@pytest.mark.parametrize("mode", ["soft_all", "hard_range", "levels"])
+def test_dwt_inverse_thresholded(filt, n, nlev, mode):
+    """The threshold fused into every inverse pass (each pass reads its
+    levels' details out of its staged span, or in place for levels 1 and
+    2) against the oracle's threshold followed by its inverse."""
+    J = O.levels(n, nlev, filt)
+    x, _ = noisy_doppler(n, 3.0, seed=n + filt)
+    c = O.dwt(x, J, filt, sense=1)
+    if mode == "levels":
+        lams = np.linspace(0.05, 0.6, J)
+        ref = O.threshold_levels_(c.copy(), n, J, 0, filt, lams, 1)
+        thr = {"hardness": 1, "minlevel": 0, "maxlevel": J - 1,
+               "levels": torch.tensor(lams, device="cuda")}
+    else:
+        hard = 0 if mode == "hard_range" else 1
+        lo, hi = (1, J - 1) if mode == "hard_range" and J > 1 else (0, J - 1)
+        ref = O.threshold(c, n, J, 0, filt, 0.3, hard, lo, hi)
+        thr = {"value": 0.3, "hardness": hard, "minlevel": lo, "maxlevel": hi}
+    xr_ref = O.dwt(ref, J, filt, sense=0)
+    xr = D.dwt_inv(_dev(c), J, filt, thresh=thr).cpu().numpy()
+    assert O.relerr(xr, xr_ref) <= TOL64
+
+
+@pytest.mark.parametrize("filt", FILTERS)
+@pytest.mark.parametrize("n,nlev", [(1024, 0), (4096, 2), (16384, 0), (1 << 17, 6),
+                                    (1 << 17, 7), (1 << 18, 0)])
 def test_dwt_parity_f32(filt, n, nlev):
     J = O.levels(n, nlev, filt)
     x = white_noise(n, seed=7)
