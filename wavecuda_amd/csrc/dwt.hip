@@ -32,6 +32,11 @@ namespace wh {
 namespace {
 
 constexpr int kTile = 2048;   // coefficients per tile (level j0 units)
+// Passes over at most 2^19 values (the later passes of a long transform,
+// or all of a short one) use half-size tiles: twice the workgroups for a
+// grid that would leave CUs idle, each with half the dependent work
+constexpr int kTileSmall = 1024;
+constexpr int64_t kBigTileN = int64_t(1) << 20;
 constexpr int kNT = 256;      // threads per tile workgroup
 constexpr int kNTW = 1024;    // threads of the whole-sequence workgroup
 constexpr int kGmax = 5;      // levels per tile pass
@@ -86,11 +91,11 @@ struct TileArgs {
 };
 
 // ------------------------------------------------------------ forward tile
-template <typename T, int FILT>
+template <typename T, int FILT, int TILE>
 __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
   constexpr int L = Filter<FILT>::L;
   constexpr int HMAX = halo_fwd(kGmax, L);
-  constexpr int SZA = kTile + HMAX;
+  constexpr int SZA = TILE + HMAX;
   // outputs per thread at level 1 (the widest level); each level's s is
   // computed into registers and written back over its input after a
   // barrier, so one buffer serves every level.  The level-1 details stay in
@@ -98,13 +103,13 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
   // d_1[t + k kNT] and writes y positions 2 (t + k kNT) and the next), so
   // LDS holds only the deeper levels' details: 26 KB in FP64, six
   // workgroups per CU.
-  constexpr int kPer = (kTile / 2 + halo_fwd(kGmax - 1, L) + kNT - 1) / kNT;
-  constexpr int kPer1 = kTile / 2 / kNT;  // level-1 details a thread keeps
-  static_assert(kPer1 * kNT == kTile / 2, "whole level-1 pairs per thread");
+  constexpr int kPer = (TILE / 2 + halo_fwd(kGmax - 1, L) + kNT - 1) / kNT;
+  constexpr int kPer1 = TILE / 2 / kNT;  // level-1 details a thread keeps
+  static_assert(kPer1 * kNT == TILE / 2, "whole level-1 pairs per thread");
   using V = typename Vec2<T>::type;
   typedef T E2 __attribute__((ext_vector_type(2)));
   __shared__ __align__(16) T bufA[SZA + 2];
-  __shared__ __align__(16) T dbuf[kTile / 2];  // level l >= 2 at kTile/2 - (kTile >> (l-1))
+  __shared__ __align__(16) T dbuf[TILE / 2];  // level l >= 2 at TILE/2 - (TILE >> (l-1))
 
   const int tid = threadIdx.x;
   const int g = a.g;
@@ -112,8 +117,8 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
   const T* __restrict__ src = static_cast<const T*>(a.src);
   T* __restrict__ y = static_cast<T*>(a.y);
 
-  const int64_t base = int64_t(blockIdx.x) * kTile;
-  const int K0 = kTile + halo_fwd(g, L);
+  const int64_t base = int64_t(blockIdx.x) * TILE;
+  const int K0 = TILE + halo_fwd(g, L);
   // every load of the tile is issued before the first one is used (a loop
   // that stores each load to LDS before issuing the next waits one memory
   // latency per iteration); only the last tile's halo wraps around
@@ -144,9 +149,9 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
   T* cur = bufA;
   T d1[kPer1];
   for (int l = 1; l <= g; ++l) {
-    const int own = kTile >> l;
+    const int own = TILE >> l;
     const int Kl = own + halo_fwd(g - l, L);
-    const int doff = kTile / 2 - (kTile >> (l - 1));  // levels >= 2
+    const int doff = TILE / 2 - (TILE >> (l - 1));  // levels >= 2
     T sreg[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
@@ -178,7 +183,7 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
     }
     __syncthreads();
   }
-  // cur holds s_{j0+g}; own part is cur[0 .. kTile >> g).  Thread t writes
+  // cur holds s_{j0+g}; own part is cur[0 .. TILE >> g).  Thread t writes
   // the pairs of positions q = 2 (t + k kNT), q + 1: q holds a coarser
   // coefficient (a deeper detail, or s when its ctz is at least g), q + 1
   // the level-1 detail the thread kept.
@@ -189,7 +194,7 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
     const int qq = tid + k * kNT;
     const int q = 2 * qq;
     const int z = (q == 0) ? 31 : __builtin_ctz(unsigned(q));
-    const T v0 = (z >= g) ? cur[q >> g] : dbuf[(kTile / 2 - (kTile >> z)) + (q >> (z + 1))];
+    const T v0 = (z >= g) ? cur[q >> g] : dbuf[(TILE / 2 - (TILE >> z)) + (q >> (z + 1))];
     if (j0 == 0 && a.write_coarse) {
       // first pass: every position of the tile, as 16-byte pairs
       *reinterpret_cast<V*>(y + base + q) = V{v0, d1[k]};
@@ -200,7 +205,7 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
   }
   if (!a.final_pass) {
     T* __restrict__ dst = static_cast<T*>(a.dst);
-    const int own = kTile >> g;
+    const int own = TILE >> g;
     for (int i = tid; i < own; i += kNT) dst[(base >> g) + i] = cur[i];
   }
 }
@@ -221,7 +226,7 @@ __global__ __launch_bounds__(kNT) void dwt_fwd_tile(TileArgs a) {
 //      barrier), level 1 too: s_1 and d_1 sit side by side there, one
 //      16-byte read a tap, and its outputs go straight to memory.
 // 25 KB of LDS in FP64 with LA8: six workgroups per CU.
-template <typename T, int FILT>
+template <typename T, int FILT, int TILE>
 __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
   constexpr int L = Filter<FILT>::L;
   constexpr int PADS = 8;
@@ -229,14 +234,14 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
   // down to the lowest detail any of its levels reads (an even count)
   constexpr int HY = inv_halo(L);
   static_assert(HY % 2 == 0, "the span starts on a pair");
-  constexpr int kSpan = kTile + HY;
+  constexpr int kSpan = TILE + HY;
   typedef T E2 __attribute__((ext_vector_type(2)));
   // rest: d_3..d_kGmax (level l at doff(l), own_l + e_l values and PADS
   // spare), then P0 (s_4) and P1 (s_3, s_5); the top s_g of a pass with
   // g >= 3 goes where level g reads it (P0 or P1)
-  constexpr int kD = (kTile / 4 - (kTile >> kGmax)) + PADS * (kGmax - 2);
-  constexpr int kP0 = kTile / 16 + L - 2;  // s_4: own_4 + e_4
-  constexpr int kP1 = kTile / 8 + L - 2;   // s_3: own_3 + e_3
+  constexpr int kD = (TILE / 4 - (TILE >> kGmax)) + PADS * (kGmax - 2);
+  constexpr int kP0 = TILE / 16 + L - 2;  // s_4: own_4 + e_4
+  constexpr int kP1 = TILE / 8 + L - 2;   // s_3: own_3 + e_3
   static_assert(kP0 % 2 == 0 && kP1 % 2 == 0 && kD % 2 == 0, "pair-aligned buffers");
   __shared__ __align__(16) T pool[kSpan];
   __shared__ __align__(16) T rest[kD + kP0 + kP1];
@@ -246,12 +251,12 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
   const int tid = threadIdx.x;
   const int g = a.g, j0 = a.j0;
   const int64_t N = a.N;
-  const int64_t base = int64_t(blockIdx.x) * kTile;
+  const int64_t base = int64_t(blockIdx.x) * TILE;
   const T* __restrict__ src = static_cast<const T*>(a.src);
   const T* __restrict__ y = static_cast<const T*>(a.y);
 
   auto evn = [](int l) { return inv_evn(l, L); };
-  auto doff = [](int l) { return (kTile / 4 - (kTile >> (l - 1))) + PADS * (l - 3); };
+  auto doff = [](int l) { return (TILE / 4 - (TILE >> (l - 1))) + PADS * (l - 3); };
   auto sbuf = [&](int j) -> T* { return (j & 1) ? P1 : P0; };  // s_j, j >= 3
 
   // Local index i of level l's arrays (s_l, d_l) is level-l index
@@ -269,15 +274,15 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
   const uint32_t c2 = uint32_t(HY - 4 * evn(2));  // s_2[i] at slot 4i + c2, d_2[i] 2 further
   static_assert(HY >= 4 * inv_evn(2, L) && HY >= 2 * inv_evn(1, L), "s_1, s_2 inside the span");
 
-  // ---- A: the span [base - HY, base + kTile).  The first pass (j0 = 0)
+  // ---- A: the span [base - HY, base + TILE).  The first pass (j0 = 0)
   // reads it with contiguous 16-byte loads; deeper passes with one load per
   // value.  Loads past the end are clamped duplicates: a guarded load would
   // be merged with its guarded store and wait there.
   constexpr int kLdP = (kSpan / 2 + kNT - 1) / kNT;  // pair loads a thread (j0 = 0)
   constexpr int kLdS = (kSpan + kNT - 1) / kNT;      // single loads a thread (j0 > 0)
-  constexpr int kTopMax = (kTile / 2 + L + kNT - 1) / kNT;
+  constexpr int kTopMax = (TILE / 2 + L + kNT - 1) / kNT;
   const int span = int(std::min<int64_t>(kSpan, N));  // pool positions used
-  const int own_g = kTile >> g;
+  const int own_g = TILE >> g;
   const int eg = evn(g);
   const int cnt_g = own_g + eg;
   const int64_t Bg = base >> g;
@@ -370,13 +375,13 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
   // ---- B: d_3..d_g out of the span, every read issued first (for levels
   // past g too: their clamped slots stay inside the span)
   if (g >= 3) {
-    constexpr int kIt3 = (kTile / 8 + L + kNT - 1) / kNT;
+    constexpr int kIt3 = (TILE / 8 + L + kNT - 1) / kNT;
     T ev[kGmax][kIt3];
 #pragma unroll
     for (int l = 3; l <= kGmax; ++l) {
-      const int cnt = (kTile >> l) + evn(l);
+      const int cnt = (TILE >> l) + evn(l);
 #pragma unroll
-      for (int k = 0; k < ((kTile >> l) + L + kNT - 1) / kNT; ++k) {
+      for (int k = 0; k < ((TILE >> l) + L + kNT - 1) / kNT; ++k) {
         const int i = tid + k * kNT;
         ev[l - 1][k] = pool[dslot(l, i < cnt ? i : cnt - 1)];
       }
@@ -384,11 +389,11 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
 #pragma unroll
     for (int l = 3; l <= kGmax; ++l) {
       if (l <= g) {
-        const int cnt = (kTile >> l) + evn(l);
+        const int cnt = (TILE >> l) + evn(l);
         const T t = thr_level<T>(a.thr, j0 + l - 1);
         T* dl = rest + doff(l);
 #pragma unroll
-        for (int k = 0; k < ((kTile >> l) + L + kNT - 1) / kNT; ++k) {
+        for (int k = 0; k < ((TILE >> l) + L + kNT - 1) / kNT; ++k) {
           const int i = tid + k * kNT;
           if (i < cnt) {
             T d = ev[l - 1][k];
@@ -405,12 +410,12 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
   // e_l - e_{l-1}/2 + p in s_l and d_l, and outputs local 2p, 2p + 1 of
   // s_{l-1}.  Every pair is computed before any is written: the compiler
   // cannot tell the buffers apart, so a write would hold back later reads.
-  constexpr int kPairs = (kTile / 2 + kNT - 1) / kNT;
+  constexpr int kPairs = (TILE / 2 + kNT - 1) / kNT;
   for (int l = g; l >= 3; --l) {
     const T* cur = sbuf(l);
     const T* dl = rest + doff(l);
     const int elm = evn(l - 1);
-    const int npairs = ((kTile >> (l - 1)) + elm) / 2;
+    const int npairs = ((TILE >> (l - 1)) + elm) / 2;
     const int off = evn(l) - elm / 2;
     E2 res[kPairs];
 #pragma unroll
@@ -451,7 +456,7 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
   // ---- level 2 in place: read every tap, barrier, write s_1 over them
   if (g >= 2) {
     const int elm = evn(1);
-    const int npairs = ((kTile >> 1) + elm) / 2;
+    const int npairs = ((TILE >> 1) + elm) / 2;
     const int off = evn(2) - elm / 2;
     const T t = thr_level<T>(a.thr, j0 + 1);
     E2 res[kPairs];
@@ -486,7 +491,7 @@ __global__ __launch_bounds__(kNT) void dwt_inv_tile(TileArgs a) {
 
   // ---- D: level 1 in place in the span; outputs x[base + 2p], x[base + 2p + 1]
   {
-    const int npairs = kTile / 2;  // e_0 = 0
+    const int npairs = TILE / 2;  // e_0 = 0
     const int off = evn(1);
     const T t = thr_level<T>(a.thr, j0);
     T* __restrict__ dst = static_cast<T*>(a.dst);
@@ -730,8 +735,12 @@ int dwt_fwd_f(const void* x, void* y, int64_t n, int J, hipStream_t st) {
       a.g = P.g;
       a.final_pass = (P.j0 + P.g == J);
       a.write_coarse = (P.j0 == 0);
-      const int64_t blocks = P.N / kTile;
-      hipLaunchKernelGGL((dwt_fwd_tile<T, FILT>), dim3(unsigned(blocks)), dim3(kNT), 0, st, a);
+      if (P.N >= kBigTileN)
+        hipLaunchKernelGGL((dwt_fwd_tile<T, FILT, kTile>), dim3(unsigned(P.N / kTile)), dim3(kNT),
+                           0, st, a);
+      else
+        hipLaunchKernelGGL((dwt_fwd_tile<T, FILT, kTileSmall>), dim3(unsigned(P.N / kTileSmall)),
+                           dim3(kNT), 0, st, a);
       WH_HIP_CHECK(hipGetLastError());
       src = out;
       slot ^= 1;
@@ -789,8 +798,12 @@ int dwt_inv_f(const void* yin, void* x, int64_t n, int J, const wh_thresh* thr,
       a.j0 = P.j0;
       a.g = P.g;
       a.thr = th;
-      int64_t blocks = P.N / kTile;
-      hipLaunchKernelGGL((dwt_inv_tile<T, FILT>), dim3(unsigned(blocks)), dim3(kNT), 0, st, a);
+      if (P.N >= kBigTileN)
+        hipLaunchKernelGGL((dwt_inv_tile<T, FILT, kTile>), dim3(unsigned(P.N / kTile)), dim3(kNT),
+                           0, st, a);
+      else
+        hipLaunchKernelGGL((dwt_inv_tile<T, FILT, kTileSmall>), dim3(unsigned(P.N / kTileSmall)),
+                           dim3(kNT), 0, st, a);
       WH_HIP_CHECK(hipGetLastError());
     }
     src = out;
