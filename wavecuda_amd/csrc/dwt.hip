@@ -37,6 +37,8 @@ constexpr int kTile = 2048;   // coefficients per tile (level j0 units)
 // grid that would leave CUs idle, each with half the dependent work
 constexpr int kTileSmall = 1024;
 constexpr int64_t kBigTileN = int64_t(1) << 20;
+constexpr int kTileTiny = 512;  // passes over at most 2^15 values
+constexpr int64_t kSmallTileN = int64_t(1) << 16;
 constexpr int kNT = 256;      // threads per tile workgroup
 constexpr int kNTW = 1024;    // threads of the whole-sequence workgroup
 constexpr int kGmax = 5;      // levels per tile pass
@@ -738,8 +740,11 @@ int dwt_fwd_f(const void* x, void* y, int64_t n, int J, hipStream_t st) {
       if (P.N >= kBigTileN)
         hipLaunchKernelGGL((dwt_fwd_tile<T, FILT, kTile>), dim3(unsigned(P.N / kTile)), dim3(kNT),
                            0, st, a);
-      else
+      else if (P.N >= kSmallTileN)
         hipLaunchKernelGGL((dwt_fwd_tile<T, FILT, kTileSmall>), dim3(unsigned(P.N / kTileSmall)),
+                           dim3(kNT), 0, st, a);
+      else
+        hipLaunchKernelGGL((dwt_fwd_tile<T, FILT, kTileTiny>), dim3(unsigned(P.N / kTileTiny)),
                            dim3(kNT), 0, st, a);
       WH_HIP_CHECK(hipGetLastError());
       src = out;
@@ -801,8 +806,11 @@ int dwt_inv_f(const void* yin, void* x, int64_t n, int J, const wh_thresh* thr,
       if (P.N >= kBigTileN)
         hipLaunchKernelGGL((dwt_inv_tile<T, FILT, kTile>), dim3(unsigned(P.N / kTile)), dim3(kNT),
                            0, st, a);
-      else
+      else if (P.N >= kSmallTileN)
         hipLaunchKernelGGL((dwt_inv_tile<T, FILT, kTileSmall>), dim3(unsigned(P.N / kTileSmall)),
+                           dim3(kNT), 0, st, a);
+      else
+        hipLaunchKernelGGL((dwt_inv_tile<T, FILT, kTileTiny>), dim3(unsigned(P.N / kTileTiny)),
                            dim3(kNT), 0, st, a);
       WH_HIP_CHECK(hipGetLastError());
     }
