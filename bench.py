@@ -53,6 +53,31 @@ def modwt_bytes_inv(n, J, esz=8):
     return esz * n * (J + 2)
 
 
+def parse_knobs(text: str) -> tuple:
+    """'g1max,first_per_cu,deep_per_cu,threads' -> the planner's knob tuple."""
+    k = tuple(int(v) for v in text.split(","))
+    if len(k) != 4:
+        raise SystemExit(f"plan knobs {text!r}: expected four comma-separated integers")
+    return k
+
+
+def plan_knobs(D, inverse: bool, n: int, J: int, filt: int) -> str | None:
+    """The knobs (as parse_knobs reads them) of the MODWT plan now in force
+    for this shape, found among the tuner's candidates; the plan itself is
+    left as it was."""
+    current = D.modwt_plan(inverse, n, J, filt)
+    cands = D.modwt_plan_candidates(inverse, n, J, filt)  # resets to the default rules
+    k = cands.get(current)
+    default = D.modwt_plan(inverse, n, J, filt)
+    if current != default:
+        if k is None:
+            raise SystemExit("the MODWT plan in force is not among the tuner's candidates")
+        D.modwt_set_plan(inverse, n, J, filt, knobs=k)
+    if D.modwt_plan(inverse, n, J, filt) != current:
+        raise SystemExit("could not restore the MODWT plan in force")
+    return None if k is None else ",".join(str(v) for v in k)
+
+
 def plan_kernels(plan: str, inverse: bool) -> str:
     """The fused MODWT kernels a plan string launches, in launch order
     (LA8, FP64): modwt_fwd_k<T, filter, R, threads, levels, map, store> /
@@ -478,17 +503,26 @@ def host_section():
     def call(xa, xma, sense):
         W.lib().RgpuTransform(xa.ctypes.data_as(dp), xma.ctypes.data_as(dp), ip(n), ip(sense),
                               ip(J), ip(1), ip(W.LA8), ip(8))
-    call(x.copy(), xm, 1)  # warm
-    t0 = time.perf_counter()
-    call(x.copy(), xm, 1)
-    t1 = time.perf_counter()
+    # one untimed pair first: it grows the library's device buffers and
+    # touches the host arrays' pages (a caller reusing its arrays, as here,
+    # pays neither again; first-touch of fresh arrays is the caller's cost)
+    call(x, xm, 1)
     call(xr, xm, 0)
-    t2 = time.perf_counter()
+    fwd, inv = [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        call(x, xm, 1)
+        t1 = time.perf_counter()
+        call(xr, xm, 0)
+        t2 = time.perf_counter()
+        fwd.append(t1 - t0)
+        inv.append(t2 - t1)
     err = float(np.max(np.abs(xr - x)) / np.max(np.abs(x)))
+    tf, ti = sorted(fwd)[1], sorted(inv)[1]
     return {"workload": "RgpuTransform (host arrays, as R's .C passes them), LA8 MODWT J=10, "
-                        "n=2^24 FP64", "fwd_ms": round((t1 - t0) * 1e3, 1),
-            "inv_ms": round((t2 - t1) * 1e3, 1),
-            "msamples_per_s_fwd_plus_inv": round(n / (t2 - t0) / 1e6, 1),
+                        "n=2^24 FP64; median of 3 forward+inverse pairs",
+            "fwd_ms": round(tf * 1e3, 1), "inv_ms": round(ti * 1e3, 1),
+            "msamples_per_s_fwd_plus_inv": round(n / (tf + ti) / 1e6, 1),
             "reconstruction_relerr": err}
 
 
@@ -588,6 +622,11 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="use the planner's default MODWT plans instead of tuning them first")
+    ap.add_argument("--fwd-knobs", default=None,
+                    help="set the forward MODWT plan instead of tuning it: "
+                         "'g1max,first_per_cu,deep_per_cu,threads' as plan.knobs reports them")
+    ap.add_argument("--inv-knobs", default=None,
+                    help="set the inverse MODWT plan instead of tuning it (same form)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     ap.add_argument("--no-extra", action="store_true",
@@ -624,7 +663,14 @@ def main():
     xmod = torch.empty(2 * n * J, dtype=torch.float64, device="cuda")
     xr = torch.empty(n, dtype=torch.float64, device="cuda")
 
-    if not args.no_tune:
+    set_plans = args.fwd_knobs is not None or args.inv_knobs is not None
+    if set_plans:
+        # a plan fixed from the command line (counter runs repeat the plan a
+        # timed run chose: the tuner's timings under counter collection differ)
+        for inverse, text in ((False, args.fwd_knobs), (True, args.inv_knobs)):
+            if text is not None:
+                D.modwt_set_plan(inverse, n, J, filt, knobs=parse_knobs(text))
+    elif not args.no_tune:
         D.modwt_tune(n, J, filt)  # one-time plan selection, before any timing
 
     def step():
@@ -680,6 +726,7 @@ def main():
     cold_ms = sorted(cold)[len(cold) // 2]
 
     fwd_plan, inv_plan = D.modwt_plan(False, n, J, filt), D.modwt_plan(True, n, J, filt)
+    fwd_knobs, inv_knobs = plan_knobs(D, False, n, J, filt), plan_knobs(D, True, n, J, filt)
     value = n / (ms_per_step * 1e-3) / 1e6
     bf, bi = modwt_bytes_fwd(n, J), modwt_bytes_inv(n, J)
     fwd_gbs = bf / (fwd_ms * 1e-3) / 1e9
@@ -728,7 +775,9 @@ def main():
         "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
         "cold_ms_per_step": round(cold_ms, 4),
         "cold_msamples_per_s": round(n / (cold_ms * 1e-3) / 1e6, 2),
-        "plan": {"fwd": fwd_plan, "inv": inv_plan, "tuned": not args.no_tune},
+        "plan": {"fwd": fwd_plan, "inv": inv_plan,
+                 "tuned": not (args.no_tune or set_plans), "set": set_plans,
+                 "knobs": {"fwd": fwd_knobs, "inv": inv_knobs}},
         "reconstruction_relerr": err,
     }
     if not args.no_batch:

@@ -205,3 +205,50 @@ def test_batch_workload_all_signals_equal_single(dtype):
             if len(bad) > 5:
                 break
     assert not bad, f"signals differing from the single-signal path: {bad}"
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_batch_selection_routes(dtype):
+    """Detail coefficients that stress the workgroup's exact median / MAD
+    (noise; noise with one moderate or one huge spike, which spreads the
+    values' exponents; integer-valued signals, whose details tie; a
+    constant and a signed-zero signal, whose details are (nearly) all
+    equal) at 2^10, 2^14 and 2^16 samples must give thresholds and outputs
+    bit for bit equal to the single-signal path, and on the noise-like
+    signals the oracle's thresholds."""
+    rng = np.random.Generator(np.random.PCG64(77))
+    sigs, kinds = [], []
+    for n in (1 << 10, 1 << 14, 1 << 16):
+        noise = rng.standard_normal(n)
+        spike = noise.copy()
+        spike[n // 3] = 60.0
+        huge = noise.copy()
+        huge[n // 5] = 1.0e6
+        ties = np.round(rng.standard_normal(n))
+        const = np.full(n, 3.0)
+        zeros = np.where(rng.random(n) < 0.5, -0.0, 0.0)
+        for name, s in (("noise", noise), ("spike", spike), ("huge", huge), ("ties", ties),
+                        ("const", const), ("zeros", zeros)):
+            sigs.append(s)
+            kinds.append((name, n))
+    count = len(sigs)
+    lens = np.array([k[1] for k in kinds], np.int32)
+    filters = np.array([(W.LA8, W.D4, W.HAAR, W.C6)[i % 4] for i in range(count)], np.int32)
+    ttypes = np.array([(i // 2) % 2 for i in range(count)], np.int32)
+    levels = np.array([0 if t == 0 else 6 for t in ttypes], np.int32)
+    x = np.concatenate(sigs)
+    plan = BatchPlan(lens, filters, ttypes, levels, dtype=dtype)
+    xg = torch.tensor(x, dtype=dtype, device="cuda")
+    y, t = plan.smooth(xg)
+    tc = t.cpu().numpy()
+    o = plan.offsets
+    for i in range(count):
+        case = f"signal {i}: {kinds[i][0]} n={kinds[i][1]} filter={filters[i]} ttype={ttypes[i]}"
+        yi, ti = D.smooth(xg[o[i]:o[i + 1]], int(levels[i]), int(ttypes[i]), int(filters[i]),
+                          W.UNIV)
+        assert torch.equal(yi, y[o[i]:o[i + 1]]), case
+        assert ti == tc[i], case
+        if dtype == torch.float64 and kinds[i][0] in ("noise", "spike", "huge"):
+            _, ts = oracle_smooth(x[o[i]:o[i + 1]], int(lens[i]), filters[i], ttypes[i],
+                                  levels[i], W.SOFT)
+            assert same_t(tc[i], ts), case
