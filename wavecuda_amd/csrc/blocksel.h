@@ -54,11 +54,37 @@ constexpr size_t block_sel_lds() {
   return size_t(kSelCap) * 8 + size_t(2) * kSelBins * 4 + size_t(NT) * 4 + 128;
 }
 
-// f(key) for every value v[off + e * stride], e < count (|v - c| when
-// deviation), kSelUnroll loads in flight per thread
+// f(key) for every value v[off + e * stride], e < count (|v - center| when
+// deviation).  Contiguous values (stride 1, v + off aligned to two
+// elements) are read as element pairs, one 2-element vector load each;
+// kSelUnroll loads in flight per thread either way.
 template <typename T, int NT, typename F>
 __device__ __forceinline__ void sel_for_each(const T* __restrict__ v, int64_t off, int64_t stride,
                                              int64_t count, bool deviation, T c, F f) {
+  typedef T T2 __attribute__((ext_vector_type(2)));
+  auto key_of = [&](T y) { return sel_key(deviation ? T(fabs(y - c)) : y); };
+  if (stride == 1 && reinterpret_cast<uintptr_t>(v + off) % sizeof(T2) == 0) {
+    const T2* __restrict__ v2 = reinterpret_cast<const T2*>(v + off);
+    const int64_t np = count >> 1;
+    for (int64_t b = threadIdx.x; b < np; b += int64_t(NT) * kSelUnroll) {
+      T2 x[kSelUnroll];
+#pragma unroll
+      for (int u = 0; u < kSelUnroll; ++u) {
+        const int64_t p = b + int64_t(u) * NT;
+        x[u] = v2[p < np ? p : b];
+      }
+#pragma unroll
+      for (int u = 0; u < kSelUnroll; ++u) {
+        const int64_t p = b + int64_t(u) * NT;
+        if (p < np) {
+          f(key_of(x[u].x));
+          f(key_of(x[u].y));
+        }
+      }
+    }
+    if ((count & 1) && threadIdx.x == 0) f(key_of(v[off + count - 1]));
+    return;
+  }
   for (int64_t b = threadIdx.x; b < count; b += int64_t(NT) * kSelUnroll) {
     T x[kSelUnroll];
 #pragma unroll
@@ -69,11 +95,7 @@ __device__ __forceinline__ void sel_for_each(const T* __restrict__ v, int64_t of
 #pragma unroll
     for (int u = 0; u < kSelUnroll; ++u) {
       const int64_t e = b + int64_t(u) * NT;
-      if (e < count) {
-        T y = x[u];
-        if (deviation) y = fabs(y - c);
-        f(sel_key(y));
-      }
+      if (e < count) f(key_of(x[u]));
     }
   }
 }
