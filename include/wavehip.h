@@ -368,6 +368,12 @@ int wh_batch_replay(wh_batch* plan, void* stream);
    tuning run (1) or the default rule (0). Any pointer may be NULL. */
 int wh_batch_info(const wh_batch* plan, int64_t* total, int* threads, int* workgroups,
                   int64_t* lds_bytes, int64_t* workspace_bytes, int* tuned);
+/* Diagnostics (no counterpart in the reference): set the plan's launch
+   shape as wh_batch_tune would -- threads per signal 128, 256 or 512 and
+   1..the resident maximum of workgroups -- growing the workspace if needed
+   and dropping a captured graph.  Results never depend on the shape; this
+   lets a caller run and check every shape the tuner can pick. */
+int wh_debug_batch_set_shape(wh_batch* plan, int threads, int workgroups);
 
 #ifdef __cplusplus
 }

@@ -252,3 +252,33 @@ def test_batch_selection_routes(dtype):
             _, ts = oracle_smooth(x[o[i]:o[i + 1]], int(lens[i]), filters[i], ttypes[i],
                                   levels[i], W.SOFT)
             assert same_t(tc[i], ts), case
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_batch_every_launch_shape(dtype):
+    """Every launch shape the tuner can pick (128, 256 or 512 threads per
+    signal; a few workgroups or one per signal) gives the default shape's
+    outputs and thresholds bit for bit, and those equal the single-signal
+    path's."""
+    x, lens, filters, ttypes, levels = mixed_table(count=64, seed=21)
+    plan = BatchPlan(lens, filters, ttypes, levels, dtype=dtype)
+    xg = torch.tensor(x, dtype=dtype, device="cuda")
+    y0, t0 = plan.smooth(xg)
+    o = plan.offsets
+    for i in range(len(lens)):
+        yi, ti = D.smooth(xg[o[i]:o[i + 1]], int(levels[i]), int(ttypes[i]), int(filters[i]),
+                          W.UNIV)
+        assert torch.equal(yi, y0[o[i]:o[i + 1]]) and ti == t0[i].item(), f"signal {i}"
+    for nt in (128, 256, 512):
+        for wg in (3, len(lens)):
+            plan.set_shape(nt, wg)
+            info = plan.info()
+            assert info["threads"] == nt and info["workgroups"] == wg
+            y, t = plan.smooth(xg)
+            case = f"{nt} threads x {wg} workgroups"
+            assert torch.equal(y, y0), case
+            assert torch.equal(t, t0), case
+    with pytest.raises(W.WaveHipError):
+        plan.set_shape(384, 3)
+    with pytest.raises(W.WaveHipError):
+        plan.set_shape(256, len(lens) + 1)

@@ -91,6 +91,7 @@ SIGNATURES = {
     "wh_batch_tune": (_I, [_VP, _VP, _VP, _VP, _VP]),
     "wh_batch_capture": (_I, [_VP, _VP, _VP, _VP]),
     "wh_batch_replay": (_I, [_VP, _VP]),
+    "wh_debug_batch_set_shape": (_I, [_VP, _I, _I]),
     "wh_batch_info": (_I, [_VP, C.POINTER(_I64), _IP, _IP, C.POINTER(_I64), C.POINTER(_I64),
                            _IP]),
 }

@@ -99,6 +99,12 @@ class BatchPlan:
             thresh = torch.empty(self.count, dtype=torch.float64, device=x.device)
         return y, thresh
 
+    def set_shape(self, threads: int, workgroups: int) -> None:
+        """Launch shape as the tuner would set it (diagnostics: results never
+        depend on the shape)."""
+        W.check(W.lib().wh_debug_batch_set_shape(self._h, int(threads), int(workgroups)),
+                "wh_debug_batch_set_shape")
+
     def smooth(self, x, y=None, thresh=None):
         """Smoothed signals (packed like x) and the per-signal thresholds."""
         self._check_x(x)

@@ -2,6 +2,8 @@
 // (item table, workspace, launch shape), the autotuner and the captured
 // graph.  The kernel is in batch_kern.h, instantiated in batch_<type>.hip.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstdint>
 #include <map>
@@ -219,9 +221,13 @@ int wh_batch_create(const int* lens, const int* filters, const int* ttypes, cons
     }
   }
   if (!p->tuned) {
-    int rc = max_grid(p, 256, &p->grid);
+    // untuned: the shape the tuner picks for mixed batches on MI355X (one
+    // 512-thread workgroup per CU in FP64; in FP32 128 threads and every
+    // workgroup that fits); wh_batch_tune times the others
+    const int NT = dtype == WH_F32 ? 128 : 512;
+    int rc = max_grid(p, NT, &p->grid);
     if (rc) return fail(rc);
-    p->NT = 256;
+    p->NT = NT;
   }
   if (int rc = ensure_scratch(p, p->grid)) return fail(rc);
   *out = p;
@@ -277,6 +283,7 @@ int wh_batch_tune(wh_batch* p, const void* x, void* y, double* thresh_out, void*
   float best = 1e30f;
   Cand bc = {p->NT, p->grid};
   int rc = WH_OK;
+  const bool log = std::getenv("WH_TUNE_LOG") != nullptr;  // candidate timings to stderr
   for (const Cand& c : cands) {
     if ((rc = run(p, c.NT, c.grid, x, y, thresh_out, st))) break;  // warm-up
     float tmin = 1e30f;
@@ -289,6 +296,7 @@ int wh_batch_tune(wh_batch* p, const void* x, void* y, double* thresh_out, void*
       if (!rc && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) tmin = std::min(tmin, ms);
     }
     if (rc) break;
+    if (log) std::fprintf(stderr, "wh_batch_tune: %d threads x %u workgroups: %.3f ms\n", c.NT, c.grid, tmin);
     if (tmin < best) {
       best = tmin;
       bc = c;
@@ -356,6 +364,21 @@ int wh_batch_info(const wh_batch* p, int64_t* total, int* threads, int* workgrou
   if (lds_bytes) *lds_bytes = int64_t(p->lds[nt_index(p->NT)]);
   if (workspace_bytes) *workspace_bytes = int64_t(p->scratch_bytes);
   if (tuned) *tuned = p->tuned;
+  return WH_OK;
+}
+
+int wh_debug_batch_set_shape(wh_batch* p, int threads, int workgroups) {
+  if (!p) return set_error(WH_ERR_ARG, "null plan");
+  if (threads != 128 && threads != 256 && threads != 512)
+    return set_error(WH_ERR_ARG, "batch: threads must be 128, 256 or 512");
+  unsigned gmax = 0;
+  if (int rc = max_grid(p, threads, &gmax)) return rc;
+  if (workgroups < 1 || unsigned(workgroups) > gmax)
+    return set_error(WH_ERR_ARG, "batch: workgroups must be 1..the resident maximum");
+  if (int rc = ensure_scratch(p, unsigned(workgroups))) return rc;
+  p->NT = threads;
+  p->grid = unsigned(workgroups);
+  drop_graph(p);
   return WH_OK;
 }
 
