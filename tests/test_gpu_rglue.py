@@ -160,11 +160,14 @@ def test_errors_leave_outputs_untouched(capfd):
     assert err.count("Error in GPU") == 3
 
 
-def test_large_transfer_matches_device_path():
-    """LA8 MODWT J = 10 at n = 2^22 through the staging ring (the 671 MB
-    coefficient array crosses in 21 chunks each way): identical to the
-    device-pointer path bit for bit."""
-    n, J, f = 1 << 22, 10, 4
+@pytest.mark.parametrize("n,J", [(1 << 22, 10), (1 << 20, 5)])
+def test_large_transfer_matches_device_path(n, J):
+    """LA8 MODWT through the staging ring of 32 MiB slots, identical to the
+    device-pointer path bit for bit: J = 10 at n = 2^22 (the 640 MiB
+    coefficient array crosses in 20 full chunks each way, x in one), J = 5
+    at n = 2^20 (80 MiB: two full chunks and a half one; x in a partial
+    first chunk)."""
+    f = 4
     x = np.random.Generator(np.random.PCG64(5)).standard_normal(n)
     xmod = np.zeros(2 * n * J)
     transform("RcpuTransform", x.copy(), xmod, n, 1, J, 1, f)
