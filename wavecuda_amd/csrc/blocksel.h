@@ -55,34 +55,36 @@ constexpr size_t block_sel_lds() {
 }
 
 // f(key) for every value v[off + e * stride], e < count (|v - center| when
-// deviation).  Contiguous values (stride 1, v + off aligned to two
-// elements) are read as element pairs, one 2-element vector load each;
+// deviation).  Contiguous values (stride 1, v + off aligned to 16 bytes)
+// are read 16 bytes at a time (two FP64 or four FP32 values per load);
 // kSelUnroll loads in flight per thread either way.
 template <typename T, int NT, typename F>
 __device__ __forceinline__ void sel_for_each(const T* __restrict__ v, int64_t off, int64_t stride,
                                              int64_t count, bool deviation, T c, F f) {
-  typedef T T2 __attribute__((ext_vector_type(2)));
+  constexpr int kW = 16 / int(sizeof(T));
+  typedef T TW __attribute__((ext_vector_type(kW)));
   auto key_of = [&](T y) { return sel_key(deviation ? T(fabs(y - c)) : y); };
-  if (stride == 1 && reinterpret_cast<uintptr_t>(v + off) % sizeof(T2) == 0) {
-    const T2* __restrict__ v2 = reinterpret_cast<const T2*>(v + off);
-    const int64_t np = count >> 1;
-    for (int64_t b = threadIdx.x; b < np; b += int64_t(NT) * kSelUnroll) {
-      T2 x[kSelUnroll];
+  if (stride == 1 && reinterpret_cast<uintptr_t>(v + off) % sizeof(TW) == 0) {
+    const TW* __restrict__ vw = reinterpret_cast<const TW*>(v + off);
+    const int64_t nv = count / kW;
+    for (int64_t b = threadIdx.x; b < nv; b += int64_t(NT) * kSelUnroll) {
+      TW x[kSelUnroll];
 #pragma unroll
       for (int u = 0; u < kSelUnroll; ++u) {
         const int64_t p = b + int64_t(u) * NT;
-        x[u] = v2[p < np ? p : b];
+        x[u] = vw[p < nv ? p : b];
       }
 #pragma unroll
       for (int u = 0; u < kSelUnroll; ++u) {
         const int64_t p = b + int64_t(u) * NT;
-        if (p < np) {
-          f(key_of(x[u].x));
-          f(key_of(x[u].y));
+        if (p < nv) {
+#pragma unroll
+          for (int w = 0; w < kW; ++w) f(key_of(x[u][w]));
         }
       }
     }
-    if ((count & 1) && threadIdx.x == 0) f(key_of(v[off + count - 1]));
+    const int64_t tail = nv * kW;
+    if (threadIdx.x < count - tail) f(key_of(v[off + tail + threadIdx.x]));
     return;
   }
   for (int64_t b = threadIdx.x; b < count; b += int64_t(NT) * kSelUnroll) {
