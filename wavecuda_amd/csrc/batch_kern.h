@@ -144,6 +144,7 @@ __device__ __forceinline__ void batch_dwt_fwd(const BatchArgs& b, const BatchIte
   constexpr int L = Filter<FILT>::L;
   constexpr int kBatchLds = batch_lds_elems<T>();
   constexpr int kU = 4;  // global-level pairs per thread in flight
+  typedef T T2 __attribute__((ext_vector_type(2)));
   T* lA = reinterpret_cast<T*>(smem);
   T* lB = lA + kBatchLds;
   const int tid = threadIdx.x;
@@ -167,9 +168,16 @@ __device__ __forceinline__ void batch_dwt_fwd(const BatchArgs& b, const BatchIte
         T v[kU][L];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
+          // taps 2m .. 2m + L - 1 as L / 2 aligned element pairs (N is even,
+          // so no pair straddles the wrap); clamped m, unconditional loads
           const int64_t m = m0 + int64_t(u) * NT;
+          const int64_t mc = (m < N / 2) ? m : m0;
 #pragma unroll
-          for (int k = 0; k < L; ++k) v[u][k] = (m < N / 2) ? src[(2 * m + k) & (N - 1)] : T(0);
+          for (int k = 0; k < L; k += 2) {
+            const T2 pr = *reinterpret_cast<const T2*>(src + ((2 * mc + k) & (N - 1)));
+            v[u][k] = pr.x;
+            v[u][k + 1] = pr.y;
+          }
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
@@ -203,7 +211,11 @@ __device__ __forceinline__ void batch_dwt_fwd(const BatchArgs& b, const BatchIte
       for (int m = tid; m < P; m += NT) {
         T v[L];
 #pragma unroll
-        for (int k = 0; k < L; ++k) v[k] = cur[(2 * m + k) & (N - 1)];
+        for (int k = 0; k < L; k += 2) {
+          const T2 pr = *reinterpret_cast<const T2*>(cur + ((2 * m + k) & (N - 1)));
+          v[k] = pr.x;
+          v[k + 1] = pr.y;
+        }
         T s, d;
         fwd_pair<T, FILT>(v, s, d);
         nxt[m] = s;
