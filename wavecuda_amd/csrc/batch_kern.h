@@ -241,7 +241,8 @@ __device__ __forceinline__ void batch_dwt_inv(const BatchArgs& b, const BatchIte
                                               unsigned char* smem, double t) {
   constexpr int L = Filter<FILT>::L;
   constexpr int kBatchLds = batch_lds_elems<T>();
-  constexpr int kU = 4;  // global-level pairs per thread in flight
+  constexpr int kUp = 2;  // global-level output pairs of pairs per thread in flight
+  typedef T T2 __attribute__((ext_vector_type(2)));
   T* lA = reinterpret_cast<T*>(smem);
   T* lB = lA + kBatchLds;
   const int tid = threadIdx.x;
@@ -280,8 +281,7 @@ __device__ __forceinline__ void batch_dwt_inv(const BatchArgs& b, const BatchIte
         }
         T e, o;
         inv_pair<T, FILT>(sv, dv, e, o);
-        nxt[2 * m] = e;
-        nxt[2 * m + 1] = o;
+        *reinterpret_cast<T2*>(nxt + 2 * m) = T2{e, o};
       }
       __syncthreads();
       T* tp = cur;
@@ -301,30 +301,54 @@ __device__ __forceinline__ void batch_dwt_inv(const BatchArgs& b, const BatchIte
       }
       const T* s = gA;
       bool inA = true;
+      // each thread takes the outputs of m = 2p and 2p + 1: their taps
+      // 2p - H + 1 .. 2p + 1 of s_j and d_j come as K aligned element pairs
+      // (N is even here, so no pair straddles the wrap), and the four
+      // outputs go out as two 16-byte pair stores
+      constexpr int H = L / 2, K = (H + 2) / 2;
       for (; j >= 1; --j) {
         T* out = (j == 1) ? y : (inA ? gB : gA);
-        for (int64_t m0 = tid; m0 < N; m0 += int64_t(NT) * kU) {
-          T sv[kU][L / 2], dv[kU][L / 2];
+        const T* dj = C + batch_dwt_doff(n, j);
+        const int64_t np = N / 2;
+        for (int64_t p0 = tid; p0 < np; p0 += int64_t(NT) * kUp) {
+          T ws[kUp][2 * K], wd[kUp][2 * K];
 #pragma unroll
-          for (int u = 0; u < kU; ++u) {
-            const int64_t m = m0 + int64_t(u) * NT;
+          for (int u = 0; u < kUp; ++u) {
+            const int64_t p = p0 + int64_t(u) * NT;
+            const int64_t pc = (p < np) ? p : p0;  // clamped: unconditional loads
 #pragma unroll
-            for (int q = 0; q < L / 2; ++q) {
-              const int64_t mm = (m - q) & (N - 1);
-              sv[u][q] = (m < N) ? s[mm] : T(0);
-              dv[u][q] = (m < N) ? C[batch_dwt_doff(n, j) + mm] : T(0);
+            for (int k = 0; k < K; ++k) {
+              const int64_t i = (2 * pc - 2 * K + 2 + 2 * k) & (N - 1);
+              const T2 a = *reinterpret_cast<const T2*>(s + i);
+              const T2 d = *reinterpret_cast<const T2*>(dj + i);
+              ws[u][2 * k] = a.x;
+              ws[u][2 * k + 1] = a.y;
+              wd[u][2 * k] = d.x;
+              wd[u][2 * k + 1] = d.y;
             }
           }
 #pragma unroll
-          for (int u = 0; u < kU; ++u) {
-            const int64_t m = m0 + int64_t(u) * NT;
-            if (m < N) {
+          for (int u = 0; u < kUp; ++u) {
+            const int64_t p = p0 + int64_t(u) * NT;
+            if (p < np) {
 #pragma unroll
-              for (int q = 0; q < L / 2; ++q) dv[u][q] = thresh_coef(dv[u][q], tt, b.hardness);
-              T e, o;
-              inv_pair<T, FILT>(sv[u], dv[u], e, o);
-              out[2 * m] = e;
-              out[2 * m + 1] = o;
+              for (int i = 0; i < 2 * K; ++i) wd[u][i] = thresh_coef(wd[u][i], tt, b.hardness);
+              // output m = 2p + r reads s_j[m - q] = w[2K - 2 + r - q]
+              T sv[H], dv[H], e0, o0, e1, o1;
+#pragma unroll
+              for (int q = 0; q < H; ++q) {
+                sv[q] = ws[u][2 * K - 2 - q];
+                dv[q] = wd[u][2 * K - 2 - q];
+              }
+              inv_pair<T, FILT>(sv, dv, e0, o0);
+#pragma unroll
+              for (int q = 0; q < H; ++q) {
+                sv[q] = ws[u][2 * K - 1 - q];
+                dv[q] = wd[u][2 * K - 1 - q];
+              }
+              inv_pair<T, FILT>(sv, dv, e1, o1);
+              *reinterpret_cast<T2*>(out + 4 * p) = T2{e0, o0};
+              *reinterpret_cast<T2*>(out + 4 * p + 2) = T2{e1, o1};
             }
           }
         }
