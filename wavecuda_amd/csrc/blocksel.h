@@ -133,22 +133,51 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
       const uint64_t q0 = st[0], q1 = st[1];
       const long long k0 = ks[0], k1 = ks[1];
       __syncthreads();  // every thread has read st and ks before they change
-      for (int i = tid; i < nc; i += NT) {
-        const uint64_t x = cand[i], tx = x >> hi;
-        long long lt0 = 0, eq0 = 0, lt1 = 0, eq1 = 0;
-        for (int j = 0; j < nc; ++j) {
+      // counts of candidate x among the candidates j in [j0, j1)
+      auto count = [&](uint64_t x, int j0, int j1, int* c) {
+        c[0] = c[1] = c[2] = c[3] = 0;
+        for (int j = j0; j < j1; ++j) {
           const uint64_t y = cand[j], ty = y >> hi;
           if (ty == q0) {
-            lt0 += (y < x);
-            eq0 += (y == x);
+            c[0] += (y < x);
+            c[1] += (y == x);
           }
           if (ty == q1) {
-            lt1 += (y < x);
-            eq1 += (y == x);
+            c[2] += (y < x);
+            c[3] += (y == x);
           }
         }
-        if (tx == q0 && lt0 <= k0 && k0 < lt0 + eq0) st[0] = x;
-        if (tx == q1 && lt1 <= k1 && k1 < lt1 + eq1) st[1] = x;
+      };
+      auto decide = [&](uint64_t x, const int* c) {
+        const uint64_t tx = x >> hi;
+        if (tx == q0 && c[0] <= k0 && k0 < c[0] + c[1]) st[0] = x;
+        if (tx == q1 && c[2] <= k1 && k1 < c[2] + c[3]) st[1] = x;
+      };
+      if (2 * nc <= NT) {
+        // two threads per candidate, each counting half of the others
+        // (partial counts in the histograms' space, unused here)
+        int* pc = reinterpret_cast<int*>(hist);  // [4][NT]
+        const int half = tid / (NT / 2), i = tid % (NT / 2);
+        const int jm = nc / 2;
+        if (i < nc) {
+          int c[4];
+          count(cand[i], half ? jm : 0, half ? nc : jm, c);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) pc[q * NT + tid] = c[q];
+        }
+        __syncthreads();
+        if (tid < nc) {
+          int c[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) c[q] = pc[q * NT + tid] + pc[q * NT + tid + NT / 2];
+          decide(cand[tid], c);
+        }
+      } else {
+        for (int i = tid; i < nc; i += NT) {
+          int c[4];
+          count(cand[i], 0, nc, c);
+          decide(cand[i], c);
+        }
       }
       __syncthreads();
       break;
