@@ -111,7 +111,7 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
   uint64_t* cand = reinterpret_cast<uint64_t*>(ws);                     // [kSelCap]
   unsigned* hist = reinterpret_cast<unsigned*>(cand + kSelCap);        // [2][kSelBins]
   unsigned* part = hist + 2 * kSelBins;                                 // [NT]
-  uint64_t* st = reinterpret_cast<uint64_t*>(part + NT);  // prefix0, prefix1, digit0, digit1
+  uint64_t* st = reinterpret_cast<uint64_t*>(part + NT);  // prefix0, prefix1 (+2 spare)
   long long* ks = reinterpret_cast<long long*>(st + 4);   // k0, k1
   long long* cs = ks + 2;                                  // keys under the new prefixes
   int* ncand = reinterpret_cast<int*>(cs + 2);
@@ -238,7 +238,9 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
         for (int i = 0; i < per; ++i) {
           const unsigned cnt = hr[tid * per + i];
           if (acc + cnt > k) {
-            st[2 + r] = uint64_t(tid * per + i);
+            // the rank's prefix grows by this digit (every thread read the
+            // old prefixes before counting)
+            st[r] = ((r ? p1 : p0) << (hi - lo)) | uint64_t(tid * per + i);
             ks[r] = k - acc;
             cs[r] = cnt;
             break;
@@ -246,11 +248,6 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
           acc += cnt;
         }
       }
-    }
-    __syncthreads();
-    if (tid == 0) {
-      st[0] = (st[0] << (hi - lo)) | st[2];
-      st[1] = (st[1] << (hi - lo)) | st[3];
     }
     __syncthreads();
     hi = lo;
