@@ -94,9 +94,9 @@ __device__ __forceinline__ ModwtArgs batch_modwt_args(const BatchArgs& b, const 
 
 template <typename T, int FILT, int NT, int G>
 __device__ __forceinline__ void batch_modwt_fwd(const BatchArgs& b, const BatchItem& I, T* scr,
-                                                unsigned char* smem) {
+                                                unsigned char* smem, unsigned* pre) {
   const ModwtArgs a = batch_modwt_args<T, FILT, NT, G>(b, I, scr);
-  modwt_fwd_body<T, FILT, 1, NT, G, 0, kStoreSmooth>(a, 0, 0, smem);
+  modwt_fwd_body<T, FILT, 1, NT, G, 0, kStoreSmoothHist>(a, 0, 0, smem, pre);
 }
 
 // MODWT item, inverse: the fused inverse into y, thresholding every detail
@@ -124,6 +124,22 @@ constexpr int batch_lds_elems() {
   return 4096;
 }
 
+// FP64 512-thread workgroups keep a first-digit histogram of the median's
+// values (2048 counts) at this LDS offset, beyond the forward's rings (at
+// most 80 KB at J = 8), the DWT's LDS levels (64 KB) and the selection's
+// workspace; the plan sizes the launch's LDS to cover it.  Other shapes
+// would need more LDS per workgroup than they have to spare.
+constexpr size_t kBatchPreOff = size_t(96) << 10;
+constexpr size_t kBatchPreEnd = kBatchPreOff + size_t(kSelBins) * 4;
+// LDS the histogram needs for NT threads and esz-byte elements (0: none)
+__host__ __device__ constexpr size_t batch_pre_lds(int NT, size_t esz) {
+  return (esz == 8 && NT == 512) ? kBatchPreEnd : 0;
+}
+template <typename T, int NT>
+__host__ __device__ constexpr bool batch_pre() {
+  return batch_pre_lds(NT, sizeof(T)) != 0;
+}
+
 // DWT item coefficients in scratch C, level by level and each level
 // contiguous: d_j at C[n - n/2^{j-1} ..) (n/2^j values), then s_J; for
 // J = 1, s_1[0] goes last, so that C[0 .. n-1) holds d_1 and s_1[1..]: the
@@ -140,7 +156,7 @@ __device__ __forceinline__ int64_t batch_dwt_sidx(int64_t n, int J, int64_t m) {
 // scratch C as above (n values, gA/gB n/2 each behind it).
 template <typename T, int FILT, int NT>
 __device__ __forceinline__ void batch_dwt_fwd(const BatchArgs& b, const BatchItem& I, T* C,
-                                              unsigned char* smem) {
+                                              unsigned char* smem, unsigned* pre) {
   constexpr int L = Filter<FILT>::L;
   constexpr int kBatchLds = batch_lds_elems<T>();
   constexpr int kU = 4;  // global-level pairs per thread in flight
@@ -187,6 +203,7 @@ __device__ __forceinline__ void batch_dwt_fwd(const BatchArgs& b, const BatchIte
             fwd_pair<T, FILT>(v[u], s, d);
             dst[m] = s;
             C[batch_dwt_doff(n, j) + m] = d;
+            if (pre && j == 1) atomicAdd(&pre[sel_digit0(d)], 1u);
           }
         }
       }
@@ -220,6 +237,7 @@ __device__ __forceinline__ void batch_dwt_fwd(const BatchArgs& b, const BatchIte
         fwd_pair<T, FILT>(v, s, d);
         nxt[m] = s;
         C[batch_dwt_doff(n, j) + m] = d;
+        if (pre && j == 1) atomicAdd(&pre[sel_digit0(d)], 1u);
       }
       __syncthreads();
       T* t = cur;
@@ -364,20 +382,20 @@ __device__ __forceinline__ void batch_dwt_inv(const BatchArgs& b, const BatchIte
 
 template <typename T, int FILT, int NT>
 __device__ __forceinline__ void batch_fwd(const BatchArgs& b, const BatchItem& I, T* scr,
-                                          unsigned char* smem) {
+                                          unsigned char* smem, unsigned* pre) {
   if (kind_ttype(I.kind) == kDWT) {
-    batch_dwt_fwd<T, FILT, NT>(b, I, scr, smem);
+    batch_dwt_fwd<T, FILT, NT>(b, I, scr, smem, pre);
     return;
   }
   switch (I.J) {
-    case 1: batch_modwt_fwd<T, FILT, NT, 1>(b, I, scr, smem); break;
-    case 2: batch_modwt_fwd<T, FILT, NT, 2>(b, I, scr, smem); break;
-    case 3: batch_modwt_fwd<T, FILT, NT, 3>(b, I, scr, smem); break;
-    case 4: batch_modwt_fwd<T, FILT, NT, 4>(b, I, scr, smem); break;
-    case 5: batch_modwt_fwd<T, FILT, NT, 5>(b, I, scr, smem); break;
-    case 6: batch_modwt_fwd<T, FILT, NT, 6>(b, I, scr, smem); break;
-    case 7: batch_modwt_fwd<T, FILT, NT, 7>(b, I, scr, smem); break;
-    default: batch_modwt_fwd<T, FILT, NT, 8>(b, I, scr, smem); break;
+    case 1: batch_modwt_fwd<T, FILT, NT, 1>(b, I, scr, smem, pre); break;
+    case 2: batch_modwt_fwd<T, FILT, NT, 2>(b, I, scr, smem, pre); break;
+    case 3: batch_modwt_fwd<T, FILT, NT, 3>(b, I, scr, smem, pre); break;
+    case 4: batch_modwt_fwd<T, FILT, NT, 4>(b, I, scr, smem, pre); break;
+    case 5: batch_modwt_fwd<T, FILT, NT, 5>(b, I, scr, smem, pre); break;
+    case 6: batch_modwt_fwd<T, FILT, NT, 6>(b, I, scr, smem, pre); break;
+    case 7: batch_modwt_fwd<T, FILT, NT, 7>(b, I, scr, smem, pre); break;
+    default: batch_modwt_fwd<T, FILT, NT, 8>(b, I, scr, smem, pre); break;
   }
 }
 
@@ -412,22 +430,34 @@ __global__ __launch_bounds__(NT) void batch_smooth_k(BatchArgs b) {
     const int it = batch_next(b.counter, &item_sh);
     if (it >= b.count) break;
     const BatchItem I = b.items[it];
+    const int64_t n = I.n;
+    const bool dwt = kind_ttype(I.kind) == kDWT;
+    // the forward counts the first radix digit of the values the median
+    // selects from (the level-1 details; not for a one-level DWT, whose
+    // selection also takes s_1[1..]) into LDS the forward and the
+    // selection leave free
+    unsigned* pre = nullptr;
+    if constexpr (batch_pre<T, NT>()) {
+      if (!(dwt && I.J == 1)) {
+        pre = reinterpret_cast<unsigned*>(smem + kBatchPreOff);
+        for (int i = threadIdx.x; i < kSelBins; i += NT) pre[i] = 0;
+        __syncthreads();
+      }
+    }
     switch (kind_filter(I.kind)) {
-      case kHAAR: batch_fwd<T, kHAAR, NT>(b, I, scr, smem); break;
-      case kD4: batch_fwd<T, kD4, NT>(b, I, scr, smem); break;
-      case kC6: batch_fwd<T, kC6, NT>(b, I, scr, smem); break;
-      default: batch_fwd<T, kLA8, NT>(b, I, scr, smem); break;
+      case kHAAR: batch_fwd<T, kHAAR, NT>(b, I, scr, smem, pre); break;
+      case kD4: batch_fwd<T, kD4, NT>(b, I, scr, smem, pre); break;
+      case kC6: batch_fwd<T, kC6, NT>(b, I, scr, smem, pre); break;
+      default: batch_fwd<T, kLA8, NT>(b, I, scr, smem, pre); break;
     }
     batch_phase();
     // the universal threshold from the level-1 details: one call site for
     // every kind of item, so the selection is inlined once (a called
     // function would keep its saved registers in scratch memory)
-    const int64_t n = I.n;
-    const bool dwt = kind_ttype(I.kind) == kDWT;
     const int64_t moff = dwt ? 0 : n;
     const int64_t mstride = 1;
     const int64_t mcount = dwt ? (I.J == 1 ? n - 1 : n / 2) : n;
-    const double t = block_mad<T, NT>(scr, moff, mstride, mcount, smem) * I.factor;
+    const double t = block_mad<T, NT>(scr, moff, mstride, mcount, smem, pre) * I.factor;
     if (threadIdx.x == 0 && b.thresh_out) b.thresh_out[I.sig] = t;
     switch (kind_filter(I.kind)) {
       case kHAAR: batch_inv<T, kHAAR, NT>(b, I, scr, smem, t); break;

@@ -50,6 +50,7 @@ size_t item_lds(int ttype, int filt, int64_t n, int J, int NT, size_t esz) {
   const int L = filter_length(filt);
   size_t b = (NT == 512) ? block_sel_lds<512>()
              : (NT == 256 ? block_sel_lds<256>() : block_sel_lds<128>());
+  b = std::max(b, batch_pre_lds(NT, esz));  // the forward's first-digit histogram
   if (ttype == kDWT) {
     const size_t tile = (esz == 8) ? size_t(batch_lds_elems<double>()) * 8
                                    : size_t(batch_lds_elems<float>()) * 4;

@@ -28,14 +28,6 @@ constexpr int kSelCap = 2048;   // keys kept in LDS
 constexpr int kSelUnroll = 8;   // loads in flight per thread and pass
 constexpr int kSelDirect = 256; // candidates ranked by direct counting
 
-__device__ __forceinline__ uint64_t sel_key(double v) {
-  uint64_t u = __double_as_longlong(v);
-  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
-}
-__device__ __forceinline__ uint64_t sel_key(float v) {
-  uint32_t u = __float_as_uint(v);
-  return (u >> 31) ? uint64_t(~u) : uint64_t(u | 0x80000000u);
-}
 __device__ __forceinline__ double sel_val(uint64_t k, double*) {
   uint64_t u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
   return __longlong_as_double(int64_t(u));
@@ -104,10 +96,13 @@ __device__ __forceinline__ void sel_for_each(const T* __restrict__ v, int64_t of
 
 // Median of v[off + e * stride], e < count (|v - center| when deviation),
 // computed by all NT threads of the workgroup; returns the same value in
-// every thread.  ws: block_sel_lds<NT>() bytes of LDS.
+// every thread.  ws: block_sel_lds<NT>() bytes of LDS.  pre: optional
+// counts of the values' first digits (sel_digit0), kSelBins of them, which
+// then replace the first counting pass.
 template <typename T, int NT>
 __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t stride, int64_t count,
-                               bool deviation, double center, unsigned char* ws) {
+                               bool deviation, double center, unsigned char* ws,
+                               const unsigned* pre) {
   uint64_t* cand = reinterpret_cast<uint64_t*>(ws);                     // [kSelCap]
   unsigned* hist = reinterpret_cast<unsigned*>(cand + kSelCap);        // [2][kSelBins]
   unsigned* part = hist + 2 * kSelBins;                                 // [NT]
@@ -183,7 +178,10 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
       break;
     }
     const int lo = hi > kSelBits ? hi - kSelBits : 0;
-    for (int i = tid; i < 2 * kSelBins; i += NT) hist[i] = 0;
+    const bool counted = pre && hi == kbits;  // the first digit's counts are given
+    if (!counted) {
+      for (int i = tid; i < 2 * kSelBins; i += NT) hist[i] = 0;
+    }
     __syncthreads();
     const uint64_t p0 = st[0], p1 = st[1];
     const bool same = (p0 == p1);
@@ -194,7 +192,9 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
       if (top == p0) atomicAdd(&hist[d], 1u);
       if (!same && top == p1) atomicAdd(&hist[kSelBins + d], 1u);
     };
-    if (in_lds) {
+    if (counted) {
+      // nothing to count
+    } else if (in_lds) {
       for (int i = tid; i < nc; i += NT) count_key(cand[i]);
     } else {
       sel_for_each<T, NT>(v, off, stride, count, deviation, c, count_key);
@@ -208,11 +208,12 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
     constexpr int kWaves = NT / 64;
     uint64_t* wsum = reinterpret_cast<uint64_t*>(part);  // [kWaves]
     const long long k0 = ks[0], k1 = ks[1];  // the finder below updates ks
-    const unsigned* h1 = hist + (same ? 0 : kSelBins);
+    const unsigned* h0 = counted ? pre : hist;
+    const unsigned* h1 = same ? h0 : hist + kSelBins;
     uint64_t loc = 0;
 #pragma unroll
     for (int i = 0; i < per; ++i)
-      loc += uint64_t(hist[tid * per + i]) | (uint64_t(h1[tid * per + i]) << 32);
+      loc += uint64_t(h0[tid * per + i]) | (uint64_t(h1[tid * per + i]) << 32);
     uint64_t inc = loc;
     const int lane = tid & 63, wave = tid >> 6;
 #pragma unroll
@@ -229,7 +230,7 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
     const uint64_t before = base + inc - loc;
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
-      const unsigned* hr = r ? h1 : hist;
+      const unsigned* hr = r ? h1 : h0;
       const long long k = r ? k1 : k0;
       const long long bef = (long long)(r ? (before >> 32) : (before & 0xffffffffull));
       const long long mine = (long long)(r ? (loc >> 32) : (loc & 0xffffffffull));
@@ -272,11 +273,12 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
 }
 
 // 1.4826 median|v - median(v)|
+// pre: optional first-digit counts of the values (see block_median)
 template <typename T, int NT>
 __device__ double block_mad(const T* __restrict__ v, int64_t off, int64_t stride, int64_t count,
-                            unsigned char* ws) {
-  const double med = block_median<T, NT>(v, off, stride, count, false, 0.0, ws);
-  return 1.4826 * block_median<T, NT>(v, off, stride, count, true, med, ws);
+                            unsigned char* ws, const unsigned* pre = nullptr) {
+  const double med = block_median<T, NT>(v, off, stride, count, false, 0.0, ws, pre);
+  return 1.4826 * block_median<T, NT>(v, off, stride, count, true, med, ws, nullptr);
 }
 
 }  // namespace wh

@@ -194,10 +194,14 @@ __device__ __forceinline__ T synthesis(const T* ws, const T* wd, int base) {
 // kStoreAdjoint makes the forward the adjoint of the inverse: level l of the
 // group stores 2^-l D and, at the group's last level, 2^-G S, and zeros
 // in the other S slots (the inverse never reads them).  The scalings are
-// powers of two, hence exact.
+// powers of two, hence exact.  kStoreSmoothHist stores as kStoreSmooth and
+// also counts the first radix digit (sel_digit0) of every D_1 value into
+// an LDS histogram, when the body is given one (the batched pipeline's
+// median of D_1 then starts from it).
 constexpr int kStoreAll = 0;
 constexpr int kStoreSmooth = 1;
 constexpr int kStoreAdjoint = 2;
+constexpr int kStoreSmoothHist = 3;
 
 template <typename T>
 __device__ __forceinline__ void st_pol(T* p, T v, bool nt) {
@@ -216,6 +220,8 @@ struct FwdCtx {
   int M, mm, r, j0;
   int64_t n, r0, c;
   int head[G];  // ring heads (slot of position c), ring l = 0..G-1
+  unsigned* hist1;  // kStoreSmoothHist: digit counts of D_1, or null
+  bool own;         // this thread's outputs are not another thread's duplicates
 
   // level l (1-based within the group): ring l-1 -> S_{j0+l}, D_{j0+l}
   template <int l, bool STORE>
@@ -287,6 +293,12 @@ struct FwdCtx {
           } else {
             if constexpr (kS) stv(oS, V{s0, s1});
             stv(oS + n, V{d0, d1});
+          }
+          if constexpr (SP == kStoreSmoothHist && l == 1) {
+            if (hist1 && own && j0 == 0) {
+              atomicAdd(&hist1[sel_digit0(d0)], 1u);
+              atomicAdd(&hist1[sel_digit0(d1)], 1u);
+            }
           }
         } else {
           if constexpr (kS) {
@@ -364,7 +376,8 @@ struct FwdCtx {
 // workgroup of NT threads; smem holds the rings.
 template <typename T, int FILT, int R, int NT, int G, int MAP, int SP = kStoreAll>
 __device__ __forceinline__ void modwt_fwd_body(const ModwtArgs& a, int64_t rb, int64_t seg,
-                                               unsigned char* smem_raw) {
+                                               unsigned char* smem_raw,
+                                               unsigned* hist1 = nullptr) {
   using V = typename Vec2<T>::type;
   FwdCtx<T, FILT, R, G, MAP, SP> C;
   C.lds = reinterpret_cast<T*>(smem_raw);
@@ -372,8 +385,10 @@ __device__ __forceinline__ void modwt_fwd_body(const ModwtArgs& a, int64_t rb, i
   C.M = a.M;
   C.j0 = a.j0;
   C.n = a.n;
+  C.hist1 = hist1;
   // threads beyond the chunk (small signals) duplicate the work of thread
   // e mod M R and write identical values to identical places
+  C.own = 2 * int(threadIdx.x) < a.M * R;
   const int e = (2 * int(threadIdx.x)) % (a.M * R);
   C.mm = e / R;
   C.r = e % R;

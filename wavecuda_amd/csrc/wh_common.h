@@ -128,6 +128,23 @@ __device__ __forceinline__ int64_t pmod(int64_t a, int64_t n) {
   return a & (n - 1);
 }
 
+// Order-preserving integer keys of floating-point values (the radix order
+// of the exact selections): negative values have all bits flipped,
+// non-negative ones the sign bit set.  The first radix digit of a key is
+// its top kSelBits = 11 bits (key >> 53 for FP64, >> 21 for FP32).
+__device__ __forceinline__ uint64_t sel_key(double v) {
+  uint64_t u = __double_as_longlong(v);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ uint64_t sel_key(float v) {
+  uint32_t u = __float_as_uint(v);
+  return (u >> 31) ? uint64_t(~u) : uint64_t(u | 0x80000000u);
+}
+template <typename T>
+__device__ __forceinline__ unsigned sel_digit0(T v) {
+  return unsigned(sel_key(v) >> (sizeof(T) * 8 - 11));
+}
+
 }  // namespace wh
 
 #define WH_HIP_CHECK(expr)                                   \
