@@ -38,6 +38,50 @@ __device__ __forceinline__ double sel_val(uint64_t k, float*) {
   return double(__uint_as_float(u));
 }
 
+// bin of a value y in [0, hi] among kSelBins equal-width bins (scale =
+// kSelBins / hi); monotone non-decreasing in y, clamped to the last bin
+__device__ __forceinline__ int sel_bin(double y, double scale) {
+  const double u = y * scale;
+  return u > 0.0 ? (u < double(kSelBins - 1) ? int(u) : kSelBins - 1) : 0;
+}
+
+// smallest and largest digit with a non-zero count in h[0..kSelBins), by
+// all NT threads (the same in every thread; -1 and kSelBins when empty).
+// wk: 8 (NT / 64) bytes of LDS scratch.
+template <int NT>
+__device__ __forceinline__ void sel_digit_range(const unsigned* h, int* wk, int* dmin, int* dmax) {
+  constexpr int per = kSelBins / NT;
+  constexpr int kWaves = NT / 64;
+  int lo = kSelBins, hi = -1;
+#pragma unroll
+  for (int i = 0; i < per; ++i) {
+    const int d = int(threadIdx.x) * per + i;
+    if (h[d]) {
+      lo = min(lo, d);
+      hi = max(hi, d);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, __shfl_xor(lo, o, 64));
+    hi = max(hi, __shfl_xor(hi, o, 64));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    wk[wave] = lo;
+    wk[kWaves + wave] = hi;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    lo = min(lo, wk[w]);
+    hi = max(hi, wk[kWaves + w]);
+  }
+  __syncthreads();  // wk is reused by the caller
+  *dmin = lo;
+  *dmax = hi;
+}
+
 // LDS layout of block_median: the candidate keys (8-byte aligned) first,
 // then the two histograms, the scan array (the wave totals of the bucket
 // scan) and the selection state.
@@ -98,11 +142,15 @@ __device__ __forceinline__ void sel_for_each(const T* __restrict__ v, int64_t of
 // computed by all NT threads of the workgroup; returns the same value in
 // every thread.  ws: block_sel_lds<NT>() bytes of LDS.  pre: optional
 // counts of the values' first digits (sel_digit0), kSelBins of them, which
-// then replace the first counting pass.
+// then replace the first counting pass.  bound > 0 (deviations only): every
+// value lies in [0, bound]; one pass then counts the values into kSelBins
+// equal-width bins, and when the middle ranks' bins hold at most
+// kSelDirect values they are copied to LDS and ranked directly (otherwise
+// the radix digits start from the top).
 template <typename T, int NT>
 __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t stride, int64_t count,
                                bool deviation, double center, unsigned char* ws,
-                               const unsigned* pre) {
+                               const unsigned* pre, double bound) {
   uint64_t* cand = reinterpret_cast<uint64_t*>(ws);                     // [kSelCap]
   unsigned* hist = reinterpret_cast<unsigned*>(cand + kSelCap);        // [2][kSelBins]
   unsigned* part = hist + 2 * kSelBins;                                 // [NT]
@@ -119,8 +167,79 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
     *ncand = 0;
   }
   constexpr int kbits = int(sizeof(T)) * 8;
+  constexpr int per = kSelBins / NT;
+  constexpr int kWaves = NT / 64;
   bool in_lds = false;
   int nc = 0;
+  if (bound > 0.0) {
+    // equal-width bins over [0, bound] (monotone in the value): the middle
+    // ranks fall in bins b0 <= b1, with no value between them
+    const double scale = double(kSelBins) / bound;
+    for (int i = tid; i < kSelBins; i += NT) hist[i] = 0;
+    __syncthreads();
+    sel_for_each<T, NT>(v, off, stride, count, deviation, c, [&](uint64_t k) {
+      atomicAdd(&hist[sel_bin(sel_val(k, (T*)nullptr), scale)], 1u);
+    });
+    __syncthreads();
+    uint64_t* wsum = reinterpret_cast<uint64_t*>(part);  // [kWaves]
+    const long long k0 = ks[0], k1 = ks[1];
+    uint64_t loc = 0;
+#pragma unroll
+    for (int i = 0; i < per; ++i) loc += hist[tid * per + i];
+    uint64_t inc = loc;
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint64_t t = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += t;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint64_t base = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w)
+      if (w < wave) base += wsum[w];
+    const long long before = (long long)(base + inc - loc);
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const long long k = r ? k1 : k0;
+      if (before <= k && k < before + (long long)loc) {
+        long long acc = before;
+        for (int i = 0; i < per; ++i) {
+          const unsigned cnt = hist[tid * per + i];
+          if (acc + cnt > k) {
+            st[2 + r] = uint64_t(tid * per + i);
+            ks[r] = k - acc;
+            cs[r] = cnt;
+            break;
+          }
+          acc += cnt;
+        }
+      }
+    }
+    __syncthreads();
+    const int b0 = int(st[2]), b1 = int(st[3]);
+    const long long total = cs[0] + (b1 != b0 ? cs[1] : 0);
+    if (total <= kSelDirect) {
+      const long long r0 = ks[0], r1 = (b1 != b0 ? cs[0] : 0) + ks[1];
+      sel_for_each<T, NT>(v, off, stride, count, deviation, c, [&](uint64_t k) {
+        const int q = sel_bin(sel_val(k, (T*)nullptr), scale);
+        if (q >= b0 && q <= b1) cand[atomicAdd(ncand, 1)] = k;
+      });
+      __syncthreads();
+      nc = *ncand;
+      in_lds = true;
+      // every candidate is ranked among all of them (prefix 0 of 0 bits)
+      if (tid == 0) {
+        ks[0] = r0;
+        ks[1] = r1;
+      }
+    } else if (tid == 0) {  // bins too full: radix digits from the top
+      ks[0] = (count - 1) / 2;
+      ks[1] = count / 2;
+    }
+    __syncthreads();
+  }
   for (int hi = kbits; hi > 0;) {
     if (in_lds && nc <= kSelDirect) {
       // rank of each candidate among the candidates of its bucket: the key
@@ -132,7 +251,7 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
       auto count = [&](uint64_t x, int j0, int j1, int* c) {
         c[0] = c[1] = c[2] = c[3] = 0;
         for (int j = j0; j < j1; ++j) {
-          const uint64_t y = cand[j], ty = y >> hi;
+          const uint64_t y = cand[j], ty = hi >= 64 ? 0 : y >> hi;
           if (ty == q0) {
             c[0] += (y < x);
             c[1] += (y == x);
@@ -144,7 +263,7 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
         }
       };
       auto decide = [&](uint64_t x, const int* c) {
-        const uint64_t tx = x >> hi;
+        const uint64_t tx = hi >= 64 ? 0 : x >> hi;
         if (tx == q0 && c[0] <= k0 && k0 < c[0] + c[1]) st[0] = x;
         if (tx == q1 && c[2] <= k1 && k1 < c[2] + c[3]) st[1] = x;
       };
@@ -204,8 +323,6 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
     // bits, rank 1 in the high; each is at most count < 2^32, so the low
     // half never carries), so one scan serves both: each thread sums its
     // `per` buckets, a wave scans by shuffles, then the wave totals.
-    constexpr int per = kSelBins / NT;
-    constexpr int kWaves = NT / 64;
     uint64_t* wsum = reinterpret_cast<uint64_t*>(part);  // [kWaves]
     const long long k0 = ks[0], k1 = ks[1];  // the finder below updates ks
     const unsigned* h0 = counted ? pre : hist;
@@ -273,12 +390,31 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
 }
 
 // 1.4826 median|v - median(v)|
-// pre: optional first-digit counts of the values (see block_median)
+// pre: optional first-digit counts of the values (see block_median).  With
+// them (FP64) the smallest and largest digits present bound the values, so
+// the deviations' median can start from equal-width bins.
 template <typename T, int NT>
 __device__ double block_mad(const T* __restrict__ v, int64_t off, int64_t stride, int64_t count,
                             unsigned char* ws, const unsigned* pre = nullptr) {
-  const double med = block_median<T, NT>(v, off, stride, count, false, 0.0, ws, pre);
-  return 1.4826 * block_median<T, NT>(v, off, stride, count, true, med, ws, nullptr);
+  const double med = block_median<T, NT>(v, off, stride, count, false, 0.0, ws, pre, 0.0);
+  double bound = 0.0;
+  if constexpr (sizeof(T) == 8) {
+    if (pre && count > kSelDirect) {
+      int dmin, dmax;
+      sel_digit_range<NT>(pre, reinterpret_cast<int*>(ws), &dmin, &dmax);
+      if (dmin <= dmax) {
+        // every value lies in [vlo, vhi], the ends of those digits' ranges;
+        // |v - med| computed as the selection computes it (v - med rounds
+        // monotonically in v) is then at most max(vhi - med, med - vlo)
+        const T vlo = T(sel_val(uint64_t(dmin) << 53, (T*)nullptr));
+        const T vhi = T(sel_val((uint64_t(dmax) << 53) | ((uint64_t(1) << 53) - 1), (T*)nullptr));
+        const T cm = T(med);
+        const double b = double(fmax(vhi - cm, cm - vlo));
+        if (b > 0.0 && b <= 1.0e300 && double(kSelBins) / b < 1.0e300) bound = b;
+      }
+    }
+  }
+  return 1.4826 * block_median<T, NT>(v, off, stride, count, true, med, ws, nullptr, bound);
 }
 
 }  // namespace wh
