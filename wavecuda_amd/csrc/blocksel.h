@@ -84,10 +84,17 @@ __device__ __forceinline__ void sel_digit_range(const unsigned* h, int* wk, int*
 
 // LDS layout of block_median: the candidate keys (8-byte aligned) first,
 // then the two histograms, the scan array (the wave totals of the bucket
-// scan) and the selection state.
+// scan) and the selection state (prefixes, ranks, counts, the copy
+// counter, the values' first-digit range).
 template <int NT>
 constexpr size_t block_sel_lds() {
   return size_t(kSelCap) * 8 + size_t(2) * kSelBins * 4 + size_t(NT) * 4 + 128;
+}
+// where block_median leaves the smallest and largest first digit (2 ints)
+template <int NT>
+__device__ __forceinline__ int* sel_range_slot(unsigned char* ws) {
+  return reinterpret_cast<int*>(ws + size_t(kSelCap) * 8 + size_t(2) * kSelBins * 4 +
+                                size_t(NT) * 4 + 96);
 }
 
 // f(key) for every value v[off + e * stride], e < count (|v - center| when
@@ -146,8 +153,9 @@ __device__ __forceinline__ void sel_for_each(const T* __restrict__ v, int64_t of
 // value lies in [0, bound]; one pass then counts the values into kSelBins
 // equal-width bins, and when the middle ranks' bins hold at most
 // kSelDirect values they are copied to LDS and ranked directly (otherwise
-// the radix digits start from the top).
-template <typename T, int NT>
+// the radix digits start from the top).  With kRange, the smallest and
+// largest first digit any value has are left at sel_range_slot(ws).
+template <typename T, int NT, bool kRange = false>
 __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t stride, int64_t count,
                                bool deviation, double center, unsigned char* ws,
                                const unsigned* pre, double bound) {
@@ -368,6 +376,17 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
       }
     }
     __syncthreads();
+    // the first digit's counts cover every value: their extreme digits
+    // bound the values (the copy area is free until a copy pass)
+    if (kRange && hi == kbits) {
+      int dmin, dmax;
+      sel_digit_range<NT>(h0, reinterpret_cast<int*>(cand), &dmin, &dmax);
+      if (tid == 0) {
+        int* slot = sel_range_slot<NT>(ws);
+        slot[0] = dmin;
+        slot[1] = dmax;
+      }
+    }
     hi = lo;
     if (!in_lds && hi > 0) {
       const uint64_t q0 = st[0], q1 = st[1];
@@ -390,29 +409,37 @@ __device__ double block_median(const T* __restrict__ v, int64_t off, int64_t str
 }
 
 // 1.4826 median|v - median(v)|
-// pre: optional first-digit counts of the values (see block_median).  With
-// them (FP64) the smallest and largest digits present bound the values, so
-// the deviations' median can start from equal-width bins.
+// pre: optional first-digit counts of the values (see block_median).  The
+// smallest and largest first digits present (from pre or from the median's
+// own first digit) bound the values, so the deviations' median can start
+// from equal-width bins.
 template <typename T, int NT>
 __device__ double block_mad(const T* __restrict__ v, int64_t off, int64_t stride, int64_t count,
                             unsigned char* ws, const unsigned* pre = nullptr) {
-  const double med = block_median<T, NT>(v, off, stride, count, false, 0.0, ws, pre, 0.0);
+  const bool binned = count > kSelDirect;
+  // without pre the median's first digit gives the range
+  const double med =
+      (binned && !pre) ? block_median<T, NT, true>(v, off, stride, count, false, 0.0, ws, pre, 0.0)
+                       : block_median<T, NT>(v, off, stride, count, false, 0.0, ws, pre, 0.0);
+  int dr[2];
+  if (binned && pre) {
+    sel_digit_range<NT>(pre, reinterpret_cast<int*>(ws), &dr[0], &dr[1]);
+  } else {
+    const int* slot = sel_range_slot<NT>(ws);  // written before block_median's last barrier
+    dr[0] = slot[0];
+    dr[1] = slot[1];
+  }
   double bound = 0.0;
-  if constexpr (sizeof(T) == 8) {
-    if (pre && count > kSelDirect) {
-      int dmin, dmax;
-      sel_digit_range<NT>(pre, reinterpret_cast<int*>(ws), &dmin, &dmax);
-      if (dmin <= dmax) {
-        // every value lies in [vlo, vhi], the ends of those digits' ranges;
-        // |v - med| computed as the selection computes it (v - med rounds
-        // monotonically in v) is then at most max(vhi - med, med - vlo)
-        const T vlo = T(sel_val(uint64_t(dmin) << 53, (T*)nullptr));
-        const T vhi = T(sel_val((uint64_t(dmax) << 53) | ((uint64_t(1) << 53) - 1), (T*)nullptr));
-        const T cm = T(med);
-        const double b = double(fmax(vhi - cm, cm - vlo));
-        if (b > 0.0 && b <= 1.0e300 && double(kSelBins) / b < 1.0e300) bound = b;
-      }
-    }
+  if (binned && dr[0] <= dr[1]) {
+    // every value lies in [vlo, vhi], the ends of those digits' key
+    // ranges; |v - med| computed as the selection computes it (v - med
+    // rounds monotonically in v) is then at most max(vhi - med, med - vlo)
+    constexpr int sh = int(sizeof(T)) * 8 - kSelBits;
+    const T vlo = T(sel_val(uint64_t(dr[0]) << sh, (T*)nullptr));
+    const T vhi = T(sel_val((uint64_t(dr[1]) << sh) | ((uint64_t(1) << sh) - 1), (T*)nullptr));
+    const T cm = T(med);
+    const double b = double(fmax(vhi - cm, cm - vlo));
+    if (b > 0.0 && b <= 1.0e300 && double(kSelBins) / b < 1.0e300) bound = b;
   }
   return 1.4826 * block_median<T, NT>(v, off, stride, count, true, med, ws, nullptr, bound);
 }
