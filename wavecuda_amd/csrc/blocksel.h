@@ -16,6 +16,12 @@
 // holds a few dozen values after one or two digits), every candidate's rank
 // within its bucket is counted directly, one pass and one barrier instead
 // of a radix digit's five barriers for each remaining digit.
+//
+// Two shortcuts, both exact: the first digit's counts of the median's values
+// may come precomputed (the batched forward counts them as it produces the
+// level-1 details), and the deviations |v - med|, whose keys crowd a few
+// exponents, are first counted into equal-width bins over [0, bound], the
+// bound following from the extreme first digits the values have.
 #pragma once
 
 #include "wh_common.h"
