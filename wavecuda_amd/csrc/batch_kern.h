@@ -479,6 +479,17 @@ __global__ __launch_bounds__(NT) void batch_smooth_k(BatchArgs b) {
 // largest MODWT depth of one batch item
 constexpr int kBatchMaxModwtJ = 8;
 
+// the first-digit histogram at kBatchPreOff stays clear of what an FP64
+// 512-thread item puts in LDS before its selection: the deepest forward's
+// rings (LA8, J = 8, chunks of 1024), the DWT's LDS levels and the
+// selection's workspace
+static_assert((kBatchMaxModwtJ * 1024 + fwd_win_sum(8, kBatchMaxModwtJ)) * 8 <= int(kBatchPreOff),
+              "forward rings overlap the first-digit histogram");
+static_assert(2 * batch_lds_elems<double>() * 8 <= int(kBatchPreOff),
+              "DWT LDS levels overlap the first-digit histogram");
+static_assert(block_sel_lds<512>() <= kBatchPreOff,
+              "selection workspace overlaps the first-digit histogram");
+
 struct BatchLaunch {
   int NT;
   unsigned grid;
