@@ -159,7 +159,7 @@ __device__ __forceinline__ void batch_dwt_fwd(const BatchArgs& b, const BatchIte
                                               unsigned char* smem, unsigned* pre) {
   constexpr int L = Filter<FILT>::L;
   constexpr int kBatchLds = batch_lds_elems<T>();
-  constexpr int kU = 4;  // global-level pairs per thread in flight
+  constexpr int kUp = 2;  // global-level output pairs of pairs per thread in flight
   typedef T T2 __attribute__((ext_vector_type(2)));
   T* lA = reinterpret_cast<T*>(smem);
   T* lB = lA + kBatchLds;
@@ -179,31 +179,39 @@ __device__ __forceinline__ void batch_dwt_fwd(const BatchArgs& b, const BatchIte
     bool toA = true;
     while (j <= J && N > kBatchLds) {
       T* dst = toA ? gA : gB;
-      // kU pairs per thread in flight: loads first, then the arithmetic
-      for (int64_t m0 = tid; m0 < N / 2; m0 += int64_t(NT) * kU) {
-        T v[kU][L];
+      T* dj = C + batch_dwt_doff(n, j);
+      // each thread takes outputs m = 2p and 2p + 1, whose taps 4p ..
+      // 4p + L + 1 come as L / 2 + 1 aligned element pairs (N is a multiple
+      // of 4, so no pair straddles the wrap), and stores them as 16-byte
+      // pairs; kUp such pairs per thread in flight, clamped p for
+      // unconditional loads
+      const int64_t np = N / 4;
+      for (int64_t p0 = tid; p0 < np; p0 += int64_t(NT) * kUp) {
+        T v[kUp][L + 2];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-          // taps 2m .. 2m + L - 1 as L / 2 aligned element pairs (N is even,
-          // so no pair straddles the wrap); clamped m, unconditional loads
-          const int64_t m = m0 + int64_t(u) * NT;
-          const int64_t mc = (m < N / 2) ? m : m0;
+        for (int u = 0; u < kUp; ++u) {
+          const int64_t p = p0 + int64_t(u) * NT;
+          const int64_t pc = (p < np) ? p : p0;
 #pragma unroll
-          for (int k = 0; k < L; k += 2) {
-            const T2 pr = *reinterpret_cast<const T2*>(src + ((2 * mc + k) & (N - 1)));
+          for (int k = 0; k < L + 2; k += 2) {
+            const T2 pr = *reinterpret_cast<const T2*>(src + ((4 * pc + k) & (N - 1)));
             v[u][k] = pr.x;
             v[u][k + 1] = pr.y;
           }
         }
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-          const int64_t m = m0 + int64_t(u) * NT;
-          if (m < N / 2) {
-            T s, d;
-            fwd_pair<T, FILT>(v[u], s, d);
-            dst[m] = s;
-            C[batch_dwt_doff(n, j) + m] = d;
-            if (pre && j == 1) atomicAdd(&pre[sel_digit0(d)], 1u);
+        for (int u = 0; u < kUp; ++u) {
+          const int64_t p = p0 + int64_t(u) * NT;
+          if (p < np) {
+            T s0, d0, s1, d1;
+            fwd_pair<T, FILT>(v[u], s0, d0);
+            fwd_pair<T, FILT>(v[u] + 2, s1, d1);
+            *reinterpret_cast<T2*>(dst + 2 * p) = T2{s0, s1};
+            *reinterpret_cast<T2*>(dj + 2 * p) = T2{d0, d1};
+            if (pre && j == 1) {
+              atomicAdd(&pre[sel_digit0(d0)], 1u);
+              atomicAdd(&pre[sel_digit0(d1)], 1u);
+            }
           }
         }
       }
